@@ -1,0 +1,387 @@
+/*
+ * circuit_abi.h -- C ABI of the MI355X (gfx950) circuit hot path.
+ *
+ * The three per-piece tasks of the Legion circuit application
+ *   calc_new_currents   (apps/circuit/circuit.cc:352-413, circuit_gpu.cu:61-149,186-205)
+ *   distribute_charge   (apps/circuit/circuit.cc:415-443, circuit_gpu.cu:219-304)
+ *   update_voltages     (apps/circuit/circuit.cc:445-460,1290-1307, circuit_gpu.cu:306-368)
+ * behind the reference's task IDs, region field layouts and piece descriptor.
+ *
+ * Conventions
+ *   - Plain C types only; every function returns an int status (CKT_OK == 0)
+ *     and never aborts.  ckt_last_error() gives a thread-local message.
+ *   - Node and wire indices are absolute element indices, as ptr_t::value in
+ *     the reference (runtime/common.h:12-164): piece p owns wires
+ *     [first_wire, first_wire+num_wires) and nodes [first_node, first_node+num_nodes).
+ *   - "ctx" calls are stream-ordered on the context's own HIP stream and return
+ *     before the GPU finishes; ckt_ctx_synchronize() waits.  The accessor-form
+ *     calls (suffix _gpu) wait for completion before returning, as the
+ *     reference's host launchers do (circuit_gpu.cu:204,303,367).
+ *   - A stream argument is a hipStream_t passed as void* (NULL = null stream).
+ */
+#ifndef CIRCUIT_ABI_H
+#define CIRCUIT_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CKT_ABI_VERSION 1
+
+/* Physics constants, apps/circuit/circuit.h:42-44. */
+#define CKT_WIRE_SEGMENTS 10
+#define CKT_STEPS 10000
+#define CKT_DELTAT 1e-6
+
+/* PointerLocation, apps/circuit/circuit.h:13-17. */
+enum { CKT_PRIVATE_PTR = 0, CKT_SHARED_PTR = 1, CKT_GHOST_PTR = 2 };
+
+/* Task IDs, apps/circuit/circuit.h:19-28, and the reduction op ID, :30-32. */
+enum {
+  CKT_REGION_MAIN = 0,
+  CKT_CALC_NEW_CURRENTS = 1,
+  CKT_DISTRIBUTE_CHARGE = 2,
+  CKT_UPDATE_VOLTAGES = 3,
+  CKT_CALC_NEW_CURRENTS_NOP = 4,
+  CKT_DISTRIBUTE_CHARGE_NOP = 5,
+  CKT_UPDATE_VOLTAGES_NOP = 6,
+  CKT_SANITY_CHECK_WIRES = 7,
+  CKT_NUM_TASK_IDS = 8
+};
+enum { CKT_REDUCE_ID = 1 };
+
+/* CircuitNode, apps/circuit/circuit.h:35-40 (16 bytes). */
+typedef struct ckt_node {
+  float charge;
+  float voltage;
+  float capacitance;
+  float leakage;
+} ckt_node_t;
+
+/* CircuitWire, apps/circuit/circuit.h:46-55 (128 bytes, including the six
+ * padding floats crap..lastcrap).  in_ptr/out_ptr are ptr_t<CircuitNode>
+ * values; in_loc/out_loc hold CKT_*_PTR. */
+typedef struct ckt_wire {
+  uint32_t in_ptr, out_ptr;
+  int32_t in_loc, out_loc;
+  float inductance;
+  float resistance;
+  float current[CKT_WIRE_SEGMENTS];
+  float capacitance;
+  float voltage[CKT_WIRE_SEGMENTS - 1];
+  float pad[6];
+} ckt_wire_t;
+
+/* The fields of CircuitPiece (apps/circuit/circuit.h:63-70) that the GPU
+ * variants read (circuit_gpu.cu:195-203,360-365). */
+typedef struct ckt_piece {
+  uint32_t first_wire;
+  uint32_t num_wires;
+  uint32_t first_node;
+  uint32_t num_nodes;
+} ckt_piece_t;
+
+/* Status codes. */
+enum {
+  CKT_OK = 0,
+  CKT_ERR_INVALID_ARG = 1,   /* null pointer, bad enum, inconsistent sizes */
+  CKT_ERR_OUT_OF_RANGE = 2,  /* piece or index outside the loaded circuit */
+  CKT_ERR_NO_MEMORY = 3,     /* device or host allocation failed */
+  CKT_ERR_BUDGET = 4,        /* request exceeds the configured HBM budget */
+  CKT_ERR_HIP = 5,           /* a HIP runtime call failed */
+  CKT_ERR_NO_DEVICE = 6,     /* no usable gfx950 device */
+  CKT_ERR_STATE = 7,         /* call not valid in the object's current state */
+  CKT_ERR_UNSUPPORTED = 8,
+  CKT_ERR_NOT_FOUND = 9,     /* unknown circuit / request id */
+  CKT_ERR_CANCELLED = 10,
+  CKT_ERR_TIMEOUT = 11
+};
+
+/* Arithmetic policy of calc_new_currents.
+ *   EXACT : the reference kernel's operation order with IEEE-754 division
+ *           (circuit_gpu.cu:114-129); bit-identical to a strict FP32 CPU run.
+ *   RECIP : per-wire reciprocals (1/R, L/dt, dt/C) hoisted out of the step
+ *           loop and fused multiply-adds; voltages materialised every step.
+ *   FAST  : RECIP with the internal-voltage update substituted into the next
+ *           step's current update (a tridiagonal update of the 10 currents per
+ *           step, 28 FMAs).  Every step's currents are computed; internal
+ *           voltages are formed from the final currents.
+ * RECIP and FAST are not bit-exact; their error is checked against 1e-5. */
+enum { CKT_POLICY_EXACT = 0, CKT_POLICY_RECIP = 1, CKT_POLICY_FAST = 2 };
+
+/* Reduction mode of distribute_charge.
+ *   ATOMIC      : float atomics (LDS-privatised per piece when the piece's
+ *                 node range fits in LDS), order nondeterministic.
+ *   CSR         : deterministic per-node sum over a node->(wire,side) index in
+ *                 global (wire, in-before-out) order; bit-identical to the
+ *                 sequential CPU order of the reference task bodies.
+ *   CSR_KAHAN64 : CSR, with shared nodes accumulated in FP64 with Kahan
+ *                 compensation and rounded once to FP32. */
+enum { CKT_REDUCE_ATOMIC = 0, CKT_REDUCE_CSR = 1, CKT_REDUCE_CSR_KAHAN64 = 2 };
+
+/* Parameter profiles of the generator.
+ *   REFERENCE : load_circuit's distributions (circuit.cc:1122-1164).
+ *   STABLE    : same draws and topology, inductance scaled by 1e-6 and wire
+ *               capacitance U(0.001, 0.011), so the relaxation converges. */
+enum { CKT_PROFILE_REFERENCE = 0, CKT_PROFILE_STABLE = 1 };
+
+/* Arguments of load_circuit (circuit.cc:1071-1073; CLI circuit.cc:991-1040). */
+typedef struct ckt_gen_params {
+  int32_t num_pieces;
+  int32_t nodes_per_piece;
+  int32_t wires_per_piece;
+  int32_t pct_wire_in_piece;
+  int32_t random_seed;
+  int32_t profile; /* CKT_PROFILE_* */
+} ckt_gen_params_t;
+
+/* ------------------------------------------------------------------------ */
+/* Library / device utilities                                                */
+
+int ckt_abi_version(void);
+const char* ckt_status_string(int status);
+const char* ckt_last_error(void);
+/* Number of visible devices and whether device `device` is gfx950. */
+int ckt_device_info(int device, int* is_gfx950, int* num_cus, size_t* lds_per_block,
+                    size_t* hbm_bytes);
+/* Plain device allocations for hosts without their own allocator (the
+ * reference's runtime owns instances; lowlevel_gpu.cc:91-99). */
+int ckt_device_malloc(int device, void** ptr, size_t bytes);
+int ckt_device_free(void* ptr);
+int ckt_memcpy_htod(void* dst, const void* src, size_t bytes);
+int ckt_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int ckt_device_synchronize(int device);
+
+/* ------------------------------------------------------------------------ */
+/* Generator: load_circuit (circuit.cc:1071-1288) with O(1) node choice.      */
+
+/* Sizes the caller must provide for ckt_generate. */
+int ckt_generate_sizes(const ckt_gen_params_t* params, uint32_t* num_nodes,
+                       uint32_t* num_wires, uint32_t* num_pieces);
+/* Fills host arrays in the reference layouts: nodes[num_nodes], wires[num_wires],
+ * locator[num_nodes] (1 = private, 0 = shared, circuit.cc:1215,1219) and
+ * pieces[num_pieces]. `locator` and `pieces` may be NULL. */
+int ckt_generate(const ckt_gen_params_t* params, ckt_node_t* nodes, ckt_wire_t* wires,
+                 int32_t* locator, ckt_piece_t* pieces);
+
+/* ------------------------------------------------------------------------ */
+/* Accessor-form drop-ins: the entry points of circuit.h:83-104.             */
+/* Arguments are device pointers to instances in the reference layouts,      */
+/* indexed by absolute element index (GPU_Accessor::ref, lowlevel.h:749-833).*/
+/* `flag` is the reference's unused trailing argument; here it selects the   */
+/* policy (calc_new_currents) or must be 0.  Each call waits for completion. */
+
+/* calc_new_currents_gpu, circuit.h:83-88 / circuit_gpu.cu:186-205. */
+int ckt_calc_new_currents_gpu(const ckt_piece_t* piece, ckt_wire_t* wires,
+                              const ckt_node_t* pvt, const ckt_node_t* owned,
+                              const ckt_node_t* ghost, int flag, void* stream);
+/* distribute_charge_gpu, circuit.h:90-95 / circuit_gpu.cu:288-304.  Private
+ * nodes are reduced into pvt[].charge; shared and ghost contributions are
+ * folded into the float reduction buffers owned_fold[] / ghost_fold[]
+ * (AccessorGPUReductionFold, lowlevel.h:835-881), indexed absolutely. */
+int ckt_distribute_charge_gpu(const ckt_piece_t* piece, const ckt_wire_t* wires,
+                              ckt_node_t* pvt, float* owned_fold, float* ghost_fold,
+                              int flag, void* stream);
+/* update_voltages_gpu, circuit.h:97-101 / circuit_gpu.cu:352-368.  locator[i]
+ * != 0 selects pvt[i], else owned[i]. */
+int ckt_update_voltages_gpu(const ckt_piece_t* piece, ckt_node_t* pvt, ckt_node_t* owned,
+                            const int32_t* locator, int flag, void* stream);
+/* sanity_check_wires_gpu, circuit.h:103-104 / circuit_gpu.cu:151-217, without
+ * the hard-coded 10000/2500 sizes: counts wires whose in/out location is not
+ * a PointerLocation, whose in_ptr is outside the piece's node range, or whose
+ * out_ptr is >= num_nodes_total.  *num_bad receives the count. */
+int ckt_sanity_check_wires_gpu(const ckt_piece_t* piece, const ckt_wire_t* wires,
+                               uint32_t num_nodes_total, uint64_t* num_bad, void* stream);
+/* Reduction-instance semantics: fill a fold buffer with AccumulateCharge's
+ * identity 0.0f (circuit.cc:30) and apply it to the owner instance with
+ * AccumulateCharge::apply<true> (circuit.cc:32-36; lowlevel_dma.cc:324-358). */
+int ckt_fold_init_gpu(float* fold, uint32_t first, uint32_t count, void* stream);
+int ckt_fold_apply_gpu(ckt_node_t* owner, const float* fold, uint32_t first, uint32_t count,
+                       void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Context form: one circuit resident in HBM as structure-of-arrays.         */
+
+typedef struct ckt_ctx ckt_ctx_t;
+
+int ckt_ctx_create(int device, ckt_ctx_t** out);
+int ckt_ctx_destroy(ckt_ctx_t* ctx);
+void* ckt_ctx_stream(ckt_ctx_t* ctx);
+int ckt_ctx_synchronize(ckt_ctx_t* ctx);
+/* Upload a circuit given in the reference layouts (host memory) and build the
+ * device node->wire index used by the CSR reduction modes. */
+int ckt_ctx_load(ckt_ctx_t* ctx, const ckt_node_t* nodes, uint32_t num_nodes,
+                 const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
+                 const ckt_piece_t* pieces, uint32_t num_pieces);
+/* Generate (ckt_generate) and load in one call. */
+int ckt_ctx_generate(ckt_ctx_t* ctx, const ckt_gen_params_t* params);
+/* Download the current state back into the reference layouts (host memory;
+ * either pointer may be NULL).  Waits for the stream. */
+int ckt_ctx_store(ckt_ctx_t* ctx, ckt_node_t* nodes, ckt_wire_t* wires);
+int ckt_ctx_sizes(ckt_ctx_t* ctx, uint32_t* num_nodes, uint32_t* num_wires,
+                  uint32_t* num_pieces);
+int ckt_ctx_pieces(ckt_ctx_t* ctx, ckt_piece_t* pieces);
+
+/* The three tasks over `n` pieces of the loaded circuit in one launch each
+ * (an index-space launch, circuit.cc:250-251,276-277,298-299). */
+int ckt_calc_new_currents(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, int policy);
+int ckt_distribute_charge(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, int mode);
+int ckt_update_voltages(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n);
+int ckt_sanity_check_wires(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n,
+                           uint64_t* num_bad);
+
+/* Task-variant table keyed by the reference task IDs (circuit.cc:577-590).
+ * The *_NOP IDs are accepted and do nothing (dummy_task, circuit.cc:462-469). */
+typedef struct ckt_task_config {
+  int32_t policy;      /* CKT_POLICY_* for CALC_NEW_CURRENTS */
+  int32_t reduce_mode; /* CKT_REDUCE_* for DISTRIBUTE_CHARGE */
+} ckt_task_config_t;
+int ckt_execute_index_space(ckt_ctx_t* ctx, int task_id, const ckt_piece_t* pieces, uint32_t n,
+                            const ckt_task_config_t* config);
+
+/* The region_main loop (circuit.cc:232-335) over all pieces. */
+typedef struct ckt_loop_config {
+  int32_t policy;
+  int32_t reduce_mode;
+  int32_t sync;      /* -sync: 0 none, 1 wait after every launch, 2 NOP launches only */
+  int32_t use_graph; /* capture one loop in a hipGraph and replay it */
+  int32_t per_piece; /* 1: one launch per piece per task with a wait after each,
+                        as the reference runtime runs GPU point tasks
+                        (lowlevel_gpu.cc:117-160); 0: one launch per task */
+} ckt_loop_config_t;
+typedef struct ckt_loop_report {
+  double elapsed_s;          /* ELAPSED TIME (circuit.cc:316-318) */
+  double gflops;             /* GFLOPS with the reference's operation count (:320-334) */
+  double wire_updates_per_s; /* wires * loops / elapsed */
+  double kernel_ms[3];       /* summed event time of the three tasks (if timing is on) */
+} ckt_loop_report_t;
+int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config,
+                  ckt_loop_report_t* report);
+
+/* Tuning and test options (context and batch). */
+enum {
+  CKT_OPT_STEPS = 0,             /* inner steps per loop (default CKT_STEPS) */
+  CKT_OPT_CNC_BLOCK = 1,         /* calc_new_currents workgroup size: 64/128/256/512 */
+  CKT_OPT_PERSISTENT = 2,        /* 1: persistent tile-queue calc_new_currents */
+  CKT_OPT_WAVES_PER_SIMD = 3,    /* persistent grid: waves resident per SIMD (1..8) */
+  CKT_OPT_DSC_ATOMIC_IMPL = 4    /* ATOMIC reduction: 0 auto, 1 global atomics, 2 LDS */
+};
+int ckt_ctx_set_option(ckt_ctx_t* ctx, int option, int64_t value);
+
+/* Per-task kernel timing with HIP events on the context stream. */
+int ckt_ctx_set_timing(ckt_ctx_t* ctx, int enable);
+int ckt_ctx_kernel_time(ckt_ctx_t* ctx, int task_id, double* total_ms, uint64_t* launches);
+int ckt_ctx_reset_timing(ckt_ctx_t* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Batching: many independent circuits advanced by shared launches.           */
+
+typedef struct ckt_batch ckt_batch_t;
+
+typedef struct ckt_batch_config {
+  int32_t policy;
+  int32_t reduce_mode;
+  int32_t use_graph;        /* capture each loop's launches in a hipGraph */
+  int32_t block_threads;    /* 64..1024, multiple of 64; 0 = autotuned/default */
+  int32_t persistent;       /* 1: calc_new_currents as a persistent tile-queue kernel */
+  int32_t waves_per_simd;   /* persistent grid size = CUs * 4 * waves_per_simd */
+} ckt_batch_config_t;
+
+typedef struct ckt_batch_report {
+  double elapsed_s;
+  double wire_updates;       /* sum over circuits of wires * loops run */
+  double wire_updates_per_s;
+  double cnc_ms;             /* event time of calc_new_currents launches */
+  int32_t loops_run;         /* number of batch iterations */
+} ckt_batch_report_t;
+
+int ckt_batch_create(int device, uint64_t hbm_budget_bytes, ckt_batch_t** out);
+int ckt_batch_set_option(ckt_batch_t* b, int option, int64_t value);
+int ckt_batch_set_timing(ckt_batch_t* b, int enable);
+int ckt_batch_kernel_time(ckt_batch_t* b, int task_id, double* total_ms, uint64_t* launches);
+int ckt_batch_destroy(ckt_batch_t* b);
+/* Add a circuit that will run `num_loops` loops.  CKT_ERR_BUDGET if its device
+ * footprint does not fit in the remaining budget. */
+int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int32_t num_loops,
+                            uint32_t* id);
+int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,
+                  const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
+                  const ckt_piece_t* pieces, uint32_t num_pieces, int32_t num_loops,
+                  uint32_t* id);
+int ckt_batch_remove(ckt_batch_t* b, uint32_t id);
+int ckt_batch_count(ckt_batch_t* b, uint32_t* num_circuits, uint64_t* bytes_in_use);
+/* Device bytes a circuit of this shape occupies. */
+int ckt_footprint_bytes(uint32_t num_nodes, uint32_t num_wires, uint32_t num_pieces,
+                        uint64_t* bytes);
+/* Run every circuit through its remaining loops. */
+int ckt_batch_run(ckt_batch_t* b, const ckt_batch_config_t* config, ckt_batch_report_t* report);
+/* Run one loop of every circuit that has loops left (one batch iteration). */
+int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* active);
+int ckt_batch_loops_left(ckt_batch_t* b, uint32_t id, int32_t* loops_left);
+int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
+/* Time candidate schedules for the current batch and write the fastest into
+ * *best.  Only calc_new_currents is timed, with `probe_steps` inner steps per
+ * candidate; circuit state is restored afterwards. */
+int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
+                       ckt_batch_config_t* best, double* best_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of circuit simulations.                      */
+
+typedef struct ckt_server ckt_server_t;
+
+typedef struct ckt_server_config {
+  uint64_t hbm_budget_bytes;   /* admission budget for device state */
+  uint64_t max_active_wires;   /* cap on wires advanced per iteration (0 = none) */
+  int32_t max_active;          /* cap on resident circuits (0 = none) */
+  int32_t policy;
+  int32_t reduce_mode;
+  int32_t gen_threads;         /* host threads generating admitted circuits */
+} ckt_server_config_t;
+
+typedef struct ckt_request {
+  ckt_gen_params_t gen;
+  int32_t num_loops;
+  double arrival_s;            /* offset from the start of the trace */
+} ckt_request_t;
+
+typedef struct ckt_request_result {
+  int32_t status;              /* CKT_OK, or why the request failed */
+  uint32_t num_wires;
+  int32_t num_loops;
+  double arrival_s, admit_s, finish_s; /* seconds from trace start */
+  double final_voltage_checksum;       /* sum of node voltages at completion */
+} ckt_request_result_t;
+
+typedef struct ckt_server_report {
+  double elapsed_s;
+  double wire_updates;
+  double wire_updates_per_s;
+  double offered_wire_updates_per_s;
+  double latency_p50_s, latency_p99_s, latency_mean_s, latency_max_s;
+  int32_t completed, rejected, iterations, max_resident;
+} ckt_server_report_t;
+
+int ckt_server_create(int device, const ckt_server_config_t* config, ckt_server_t** out);
+int ckt_server_destroy(ckt_server_t* s);
+/* Replay a request trace in real time: each request becomes visible at its
+ * arrival offset, is admitted when budget and slots allow, and runs by
+ * continuous batching.  results[i] corresponds to reqs[i]. */
+int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
+                         ckt_request_result_t* results, ckt_server_report_t* report);
+/* Final state of request i of the last trace (kept when keep_state != 0). */
+int ckt_server_set_keep_state(ckt_server_t* s, int keep_state);
+int ckt_server_fetch(ckt_server_t* s, uint32_t index, ckt_node_t* nodes, ckt_wire_t* wires);
+
+/* HBM calibration kernel: copies `bytes` from src to dst on the device with a
+ * plain streaming kernel (used to scale profiler byte counters). */
+int ckt_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIRCUIT_ABI_H */

@@ -1,0 +1,310 @@
+// ckt_batch.cpp -- many independent circuits advanced by shared launches.
+//
+// Every batch iteration runs one loop (calc_new_currents, distribute_charge,
+// update_voltages) of every circuit that still has loops left, each task as
+// ONE launch whose tiles span all those circuits.  Iterations with an
+// unchanged circuit set replay one captured hipGraph.
+#include <time.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "ckt_engine.h"
+
+using namespace ckt;
+
+struct ckt_batch {
+  Engine eng;
+  ckt_batch_config_t applied{};
+  bool have_applied = false;
+};
+
+namespace {
+
+double now_s() {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+int check(ckt_batch_t* b) {
+  if (!b) return fail(CKT_ERR_INVALID_ARG, "null batch");
+  hipError_t e = hipSetDevice(b->eng.device());
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return CKT_OK;
+}
+
+int apply_config(ckt_batch_t* b, const ckt_batch_config_t& cfg) {
+  Engine& E = b->eng;
+  int st;
+  if (cfg.block_threads && (st = E.set_option(OPT_CNC_BLOCK, cfg.block_threads))) return st;
+  if ((st = E.set_option(OPT_PERSISTENT, cfg.persistent))) return st;
+  if (cfg.waves_per_simd && (st = E.set_option(OPT_WAVES_PER_SIMD, cfg.waves_per_simd))) return st;
+  return CKT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ckt_batch_create(int device, uint64_t hbm_budget_bytes, ckt_batch_t** out) {
+  if (!out) return fail(CKT_ERR_INVALID_ARG, "ckt_batch_create: null out");
+  *out = nullptr;
+  ckt_batch_t* b = new ckt_batch_t();
+  int st = b->eng.init(device, hbm_budget_bytes);
+  if (st) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return CKT_OK;
+}
+
+int ckt_batch_destroy(ckt_batch_t* b) {
+  delete b;
+  return CKT_OK;
+}
+
+int ckt_batch_set_option(ckt_batch_t* b, int option, int64_t value) {
+  int st = check(b);
+  if (st) return st;
+  return b->eng.set_option(option, value);
+}
+
+int ckt_batch_set_timing(ckt_batch_t* b, int enable) {
+  int st = check(b);
+  if (st) return st;
+  b->eng.set_timing(enable != 0);
+  return CKT_OK;
+}
+
+int ckt_batch_kernel_time(ckt_batch_t* b, int task_id, double* total_ms, uint64_t* launches) {
+  int st = check(b);
+  if (st) return st;
+  if (task_id < 0 || task_id >= CKT_NUM_TASK_IDS) return fail(CKT_ERR_INVALID_ARG, "bad task id");
+  return b->eng.kernel_time(task_id, total_ms, launches);
+}
+
+int ckt_footprint_bytes(uint32_t num_nodes, uint32_t num_wires, uint32_t num_pieces, uint64_t* bytes) {
+  if (!bytes) return fail(CKT_ERR_INVALID_ARG, "null out");
+  *bytes = SlabLayout::make(num_nodes, num_wires, num_pieces).total;
+  return CKT_OK;
+}
+
+int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int32_t num_loops,
+                            uint32_t* id) {
+  int st = check(b);
+  if (st) return st;
+  if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
+  uint32_t N, W, P;
+  if ((st = gen_validate(params, &N, &W, &P))) return st;
+  const uint64_t need = SlabLayout::make(N, W, P).total;
+  if (b->eng.budget() && b->eng.bytes_in_use() + need > b->eng.budget())
+    return fail(CKT_ERR_BUDGET, "circuit does not fit in the remaining HBM budget");
+  SlabImage img;
+  if ((st = image_from_gen(img, params))) return st;
+  return b->eng.add(img, num_loops, id);
+}
+
+int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,
+                  const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
+                  const ckt_piece_t* pieces, uint32_t num_pieces, int32_t num_loops, uint32_t* id) {
+  int st = check(b);
+  if (st) return st;
+  if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
+  SlabImage img;
+  if ((st = image_from_aos(img, nodes, num_nodes, wires, num_wires, locator, pieces, num_pieces)))
+    return st;
+  return b->eng.add(img, num_loops, id);
+}
+
+int ckt_batch_remove(ckt_batch_t* b, uint32_t id) {
+  int st = check(b);
+  if (st) return st;
+  return b->eng.remove(id);
+}
+
+int ckt_batch_count(ckt_batch_t* b, uint32_t* num_circuits, uint64_t* bytes_in_use) {
+  int st = check(b);
+  if (st) return st;
+  if (num_circuits) *num_circuits = (uint32_t)b->eng.count();
+  if (bytes_in_use) *bytes_in_use = b->eng.bytes_in_use();
+  return CKT_OK;
+}
+
+int ckt_batch_loops_left(ckt_batch_t* b, uint32_t id, int32_t* loops_left) {
+  int st = check(b);
+  if (st) return st;
+  Circuit* c = b->eng.find(id);
+  if (!c) return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(id));
+  if (loops_left) *loops_left = c->loops_left;
+  return CKT_OK;
+}
+
+int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* active) {
+  int st = check(b);
+  if (st) return st;
+  const ckt_batch_config_t def = {CKT_POLICY_FAST, CKT_REDUCE_ATOMIC, 1, 0, 0, 0};
+  const ckt_batch_config_t& cfg = config ? *config : def;
+  if ((st = apply_config(b, cfg))) return st;
+  Engine& E = b->eng;
+  std::vector<PieceRef> refs;
+  E.all_refs(refs, true);
+  if (!refs.empty()) {
+    if (cfg.use_graph) {
+      if ((st = E.run_graph(refs, cfg.policy, cfg.reduce_mode, 1))) return st;
+    } else {
+      if ((st = E.cnc(refs, cfg.policy))) return st;
+      if ((st = E.dsc(refs, cfg.reduce_mode))) return st;
+      if ((st = E.upv(refs))) return st;
+    }
+  }
+  int32_t n = 0;
+  for (Circuit* c : E.slots())
+    if (c->loops_left > 0) {
+      c->loops_left--;
+      if (c->loops_left > 0) n++;
+    }
+  if (active) *active = n;
+  return CKT_OK;
+}
+
+int ckt_batch_run(ckt_batch_t* b, const ckt_batch_config_t* config, ckt_batch_report_t* report) {
+  int st = check(b);
+  if (st) return st;
+  const ckt_batch_config_t def = {CKT_POLICY_FAST, CKT_REDUCE_ATOMIC, 1, 0, 0, 0};
+  const ckt_batch_config_t& cfg = config ? *config : def;
+  if ((st = apply_config(b, cfg))) return st;
+  Engine& E = b->eng;
+  double cnc0 = 0;
+  E.kernel_time(CKT_CALC_NEW_CURRENTS, &cnc0, nullptr);
+  if ((st = E.sync())) return st;
+  double wu = 0;
+  int32_t iters = 0;
+  const double t0 = now_s();
+  std::vector<PieceRef> refs;
+  for (;;) {
+    E.all_refs(refs, true);
+    if (refs.empty()) break;
+    // Replay the same graph while the set of active circuits is unchanged.
+    int32_t k = INT32_MAX;
+    for (Circuit* c : E.slots())
+      if (c->loops_left > 0) k = std::min(k, c->loops_left);
+    if (cfg.use_graph) {
+      if ((st = E.run_graph(refs, cfg.policy, cfg.reduce_mode, k))) return st;
+    } else {
+      for (int32_t i = 0; i < k; i++) {
+        if ((st = E.cnc(refs, cfg.policy))) return st;
+        if ((st = E.dsc(refs, cfg.reduce_mode))) return st;
+        if ((st = E.upv(refs))) return st;
+      }
+    }
+    for (Circuit* c : E.slots())
+      if (c->loops_left > 0) {
+        c->loops_left -= k;
+        wu += (double)c->num_wires * k;
+      }
+    iters += k;
+  }
+  if ((st = E.sync())) return st;
+  const double elapsed = now_s() - t0;
+  if (report) {
+    double cnc1 = 0;
+    E.kernel_time(CKT_CALC_NEW_CURRENTS, &cnc1, nullptr);
+    report->elapsed_s = elapsed;
+    report->wire_updates = wu;
+    report->wire_updates_per_s = elapsed > 0 ? wu / elapsed : 0;
+    report->cnc_ms = cnc1 - cnc0;
+    report->loops_run = iters;
+  }
+  return CKT_OK;
+}
+
+int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires) {
+  int st = check(b);
+  if (st) return st;
+  return b->eng.store(id, nodes, wires);
+}
+
+// Times calc_new_currents over every circuit of the batch for each candidate
+// schedule (workgroup size x plain grid or persistent queue x residency),
+// with a shortened inner step count, and keeps the fastest.  The wire state
+// changed by the probes is saved before and restored after.
+int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
+                       ckt_batch_config_t* best, double* best_ms) {
+  int st = check(b);
+  if (st) return st;
+  if (!best) return fail(CKT_ERR_INVALID_ARG, "null best");
+  if (probe_steps < 1) probe_steps = 500;
+  Engine& E = b->eng;
+  if (E.count() == 0) return fail(CKT_ERR_STATE, "empty batch");
+  hipStream_t s = E.stream();
+  // Save the mutable wire state (currents and internal voltages are one
+  // contiguous region of each slab).
+  std::vector<void*> saved;
+  for (Circuit* c : E.slots()) {
+    const size_t bytes = c->lay.ind - c->lay.cur;
+    void* p = nullptr;
+    CKT_HIP(hipMalloc(&p, bytes));
+    saved.push_back(p);
+    CKT_HIP(hipMemcpyAsync(p, (uint8_t*)c->slab + c->lay.cur, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  const int64_t steps0 = E.option(OPT_STEPS), block0 = E.option(OPT_CNC_BLOCK),
+                pers0 = E.option(OPT_PERSISTENT), waves0 = E.option(OPT_WAVES_PER_SIMD);
+  std::vector<PieceRef> refs;
+  E.all_refs(refs, false);
+  E.set_option(OPT_STEPS, probe_steps);
+  hipEvent_t a, z;
+  CKT_HIP(hipEventCreate(&a));
+  CKT_HIP(hipEventCreate(&z));
+  double best_t = 1e30;
+  ckt_batch_config_t bc{policy, CKT_REDUCE_ATOMIC, 1, 256, 0, 8};
+  const int blocks[4] = {64, 128, 256, 512};
+  for (int pers = 0; pers <= 1 && !st; pers++) {
+    for (int bi = 0; bi < 4 && !st; bi++) {
+      for (int waves : {4, 8}) {
+        if (!pers && waves != 8) continue;
+        E.set_option(OPT_CNC_BLOCK, blocks[bi]);
+        E.set_option(OPT_PERSISTENT, pers);
+        E.set_option(OPT_WAVES_PER_SIMD, waves);
+        double t = 1e30;
+        for (int rep = 0; rep < 2 && !st; rep++) {
+          if ((st = E.sync())) break;
+          hipEventRecord(a, s);
+          st = E.cnc(refs, policy);
+          hipEventRecord(z, s);
+          if (!st) st = E.sync();
+          float ms = 0;
+          if (!st) hipEventElapsedTime(&ms, a, z);
+          t = std::min(t, (double)ms);
+        }
+        if (!st && t < best_t) {
+          best_t = t;
+          bc.block_threads = blocks[bi];
+          bc.persistent = pers;
+          bc.waves_per_simd = waves;
+        }
+      }
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(z);
+  E.set_option(OPT_STEPS, steps0);
+  E.set_option(OPT_CNC_BLOCK, block0);
+  E.set_option(OPT_PERSISTENT, pers0);
+  E.set_option(OPT_WAVES_PER_SIMD, waves0);
+  for (size_t i = 0; i < saved.size(); i++) {
+    Circuit* c = E.slots()[i];
+    hipMemcpyAsync((uint8_t*)c->slab + c->lay.cur, saved[i], c->lay.ind - c->lay.cur,
+                   hipMemcpyDeviceToDevice, s);
+  }
+  hipStreamSynchronize(s);
+  for (void* p : saved) hipFree(p);
+  if (st) return st;
+  *best = bc;
+  if (best_ms) *best_ms = best_t;
+  return CKT_OK;
+}
+
+}  // extern "C"

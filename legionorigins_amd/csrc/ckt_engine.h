@@ -1,0 +1,166 @@
+// ckt_engine.h -- device-resident circuits and the launches over them.
+//
+// An Engine owns one HIP stream on one device and any number of circuits,
+// each held in one HBM slab as structure-of-arrays.  A launch covers a list
+// of (circuit, piece) references -- one piece is one point of the
+// reference's index-space launch (circuit.cc:250-251) -- and is cut into
+// tiles of consecutive wires or nodes.  The single-circuit context and the
+// batch are both Engines.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <map>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+#include "ckt_gen.h"
+#include "ckt_internal.h"
+
+namespace ckt {
+
+struct SlabLayout {
+  size_t cur, vol, ind, res, wcap, in_ptr, out_ptr, in_loc, out_loc;
+  size_t charge, voltage, ncap, leak, locator, csr_row, csr_ent, pieces;
+  size_t total;
+  uint32_t ws;
+  static SlabLayout make(uint32_t N, uint32_t W, uint32_t P);
+};
+
+// Host image of a slab, filled on the host and copied to HBM in one transfer.
+struct SlabImage {
+  SlabLayout lay{};
+  uint32_t N = 0, W = 0, P = 0;
+  std::vector<uint8_t> bytes;
+  std::vector<ckt_piece_t> pieces;
+  template <class T>
+  T* at(size_t off) {
+    return reinterpret_cast<T*>(bytes.data() + off);
+  }
+  void reset(uint32_t n, uint32_t w, uint32_t p);
+};
+
+int image_from_aos(SlabImage& img, const ckt_node_t* nodes, uint32_t N, const ckt_wire_t* wires,
+                   uint32_t W, const int32_t* locator, const ckt_piece_t* pieces, uint32_t P);
+int image_from_gen(SlabImage& img, const ckt_gen_params_t* params);
+
+struct Circuit {
+  uint32_t id = 0;
+  uint32_t num_nodes = 0, num_wires = 0, num_pieces = 0;
+  SlabLayout lay{};
+  void* slab = nullptr;
+  DevCircuit dev{};
+  std::vector<ckt_piece_t> pieces;
+  std::unordered_map<uint64_t, uint32_t> piece_index;  // descriptor -> index
+  bool csr_ok = false;   // the pieces partition the wires
+  bool lds_ok = false;   // node ranges of the pieces are disjoint
+  int32_t loops_left = 0;
+};
+
+struct PieceRef {
+  uint32_t slot;   // index into the engine's circuit table
+  uint32_t piece;  // index into the circuit's pieces
+};
+
+enum EngineOption {
+  OPT_STEPS = 0,
+  OPT_CNC_BLOCK = 1,
+  OPT_PERSISTENT = 2,
+  OPT_WAVES_PER_SIMD = 3,
+  OPT_DSC_ATOMIC_IMPL = 4,  // 0 auto, 1 global atomics, 2 LDS per piece
+};
+
+class Engine {
+ public:
+  Engine() = default;
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  int init(int device, uint64_t budget_bytes);
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  int num_cus() const { return num_cus_; }
+
+  // Circuits.
+  int add(SlabImage& img, int32_t loops, uint32_t* id);
+  int remove(uint32_t id);
+  Circuit* find(uint32_t id);
+  int store(uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
+  int download_voltages(uint32_t id, std::vector<float>& v);
+  size_t count() const { return slots_.size(); }
+  uint64_t bytes_in_use() const { return in_use_; }
+  uint64_t budget() const { return budget_; }
+  const std::vector<Circuit*>& slots() const { return slots_; }
+  int slot_of(uint32_t id) const;
+  // Reference to the loaded piece equal to `p` in circuit `slot`.
+  int ref_for(uint32_t slot, const ckt_piece_t& p, PieceRef* out) const;
+  void all_refs(std::vector<PieceRef>& refs, bool only_active) const;
+
+  // Task launches (asynchronous on the engine stream).
+  int cnc(const std::vector<PieceRef>& refs, int policy);
+  int dsc(const std::vector<PieceRef>& refs, int mode);
+  int upv(const std::vector<PieceRef>& refs);
+  int sanity(const std::vector<PieceRef>& refs, uint64_t* bad);
+  int nop();
+  int sync();
+
+  // One loop (cnc, dsc, upv) of `refs` captured in a graph and replayed
+  // `times` times.
+  int run_graph(const std::vector<PieceRef>& refs, int policy, int mode, int times);
+
+  int set_option(int opt, int64_t value);
+  int64_t option(int opt) const;
+
+  // Event timing per task id.
+  void set_timing(bool on) { timing_ = on; }
+  int kernel_time(int task, double* ms, uint64_t* launches);
+  void reset_timing();
+
+ private:
+  struct DevBuf {
+    void* p = nullptr;
+    uint32_t n = 0;
+  };
+  struct Pending {
+    int task;
+    hipEvent_t a, b;
+  };
+  int ensure_table();
+  int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
+  int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
+  void drop_caches();
+  int timed_begin(int task, hipEvent_t* a);
+  int timed_end(int task, hipEvent_t a);
+  int csr_nodes_ok(const std::vector<PieceRef>& refs) const;
+
+  int device_ = -1;
+  hipStream_t stream_ = nullptr;
+  int num_cus_ = 0;
+  size_t lds_limit_ = 0;
+  uint64_t budget_ = 0, in_use_ = 0;
+  uint32_t next_id_ = 1;
+  std::vector<Circuit*> slots_;
+  std::map<uint32_t, Circuit*> by_id_;
+  DevCircuit* d_table_ = nullptr;
+  size_t table_cap_ = 0;
+  bool table_dirty_ = true;
+  uint64_t generation_ = 1;
+  std::unordered_map<uint64_t, DevBuf> tile_cache_;
+  std::unordered_map<uint64_t, hipGraphExec_t> graphs_;
+  uint32_t* d_counters_ = nullptr;
+  unsigned long long* d_bad_ = nullptr;
+  int steps_ = CKT_STEPS;
+  int cnc_block_ = 256;
+  int persistent_ = 0;
+  int waves_per_simd_ = 8;
+  int dsc_atomic_impl_ = 0;
+  bool timing_ = false;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> event_pool_;
+  double acc_ms_[CKT_NUM_TASK_IDS] = {0};
+  uint64_t acc_n_[CKT_NUM_TASK_IDS] = {0};
+};
+
+}  // namespace ckt

@@ -1,0 +1,646 @@
+// ckt_kernels.hip -- gfx950 kernels of the circuit hot path.
+//
+// calc_new_currents   apps/circuit/circuit_gpu.cu:61-149 (math :103-135)
+// distribute_charge   apps/circuit/circuit_gpu.cu:219-286
+// update_voltages     apps/circuit/circuit_gpu.cu:306-350
+// sanity_check_wires  apps/circuit/circuit_gpu.cu:151-184
+//
+// Built with -ffp-contract=off: every a*b+c below is two rounded operations
+// unless written as fmaf().  Division is IEEE-754 correctly rounded (the HIP
+// default), FP32 denormals are preserved, packed-FP32 instructions are not
+// generated (-packed-fp32-ops).
+#include <hip/hip_runtime.h>
+
+#include "ckt_internal.h"
+
+namespace ckt {
+
+// float dt = DELTAT (circuit_gpu.cu:103, circuit.cc:376).
+#define CKT_DT_F ((float)CKT_DELTAT)
+
+// ---------------------------------------------------------------------------
+// The relaxation of one wire (circuit_gpu.cu:102-135).
+// On entry I[] and V[] hold wire.current and wire.voltage, vin/vout the
+// voltages of the in and out nodes; on exit they hold the values written back.
+
+template <int POLICY>
+struct Relax;
+
+// Reference order of operations; bit-identical to a strict FP32 CPU run.
+template <>
+struct Relax<CKT_POLICY_EXACT> {
+  static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
+                                             float vout, float L, float R, float C, int steps) {
+    const float dt = CKT_DT_F;
+    float ni[kSeg], nv[kSeg + 1];
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) ni[i] = I[i];
+    nv[0] = vin;
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
+    nv[kSeg] = vout;
+    for (int j = 0; j < steps; j++) {
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) ni[i] = ((nv[i + 1] - nv[i]) - (L * (ni[i] - I[i])) / dt) / R;
+#pragma unroll
+      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i] + (dt * (ni[i] - ni[i + 1])) / C;
+    }
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) I[i] = ni[i];
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) V[i] = nv[i + 1];
+  }
+};
+
+// Per-wire constants hoisted, FMAs in the loop, voltages kept every step:
+//   I_i <- h*(v_{i+1}-v_i) - g*(I_i - I0_i),  h = 1/R, g = (L/dt)/R
+//   v_k <- V0_k + b*(I_{k-1} - I_k),          b = dt/C
+template <>
+struct Relax<CKT_POLICY_RECIP> {
+  static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
+                                             float vout, float L, float R, float C, int steps) {
+    const float dt = CKT_DT_F;
+    const float h = 1.0f / R, g = (L / dt) / R, b = dt / C;
+    float c0[kSeg], ni[kSeg], nv[kSeg + 1];
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) {
+      c0[i] = g * I[i];
+      ni[i] = I[i];
+    }
+    nv[0] = vin;
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
+    nv[kSeg] = vout;
+    for (int j = 0; j < steps; j++) {
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) ni[i] = fmaf(-g, ni[i], fmaf(h, nv[i + 1] - nv[i], c0[i]));
+#pragma unroll
+      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = fmaf(b, ni[i] - ni[i + 1], V[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) I[i] = ni[i];
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) V[i] = nv[i + 1];
+  }
+};
+
+// After the first step every internal voltage is V0_k + b*(I_{k-1} - I_k),
+// so substituting it into the next current update gives, per step,
+//   I_i <- K_i + (2d - g) I_i - d (I_{i-1} + I_{i+1})   (i = 1..8)
+//   I_0 <- K_0 + (d - g) I_0 - d I_1,  I_9 <- K_9 + (d - g) I_9 - d I_8
+// with d = h*b and K_i = h*(V0 difference across segment i) + g*I0_i.
+// The first step uses the seeded voltages (and its inductive term is zero
+// because the running current still equals I0).  28 FMAs per step.
+template <>
+struct Relax<CKT_POLICY_FAST> {
+  static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
+                                             float vout, float L, float R, float C, int steps) {
+    if (steps <= 0) return;
+    const float dt = CKT_DT_F;
+    const float h = 1.0f / R, g = (L / dt) / R, b = dt / C;
+    const float d = h * b;
+    const float a_end = d - g, a_mid = 2.0f * d - g, m = -d;
+    float K[kSeg], x[kSeg];
+    {
+      float dv[kSeg];
+      dv[0] = V[0] - vin;
+#pragma unroll
+      for (int i = 1; i < kSeg - 1; i++) dv[i] = V[i] - V[i - 1];
+      dv[kSeg - 1] = vout - V[kSeg - 2];
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) {
+        K[i] = fmaf(h, dv[i], g * I[i]);
+        x[i] = h * dv[i];
+      }
+    }
+#define CKT_FAST_STEP                                                              \
+  {                                                                                \
+    float y[kSeg];                                                                 \
+    y[0] = fmaf(a_end, x[0], fmaf(m, x[1], K[0]));                                 \
+    _Pragma("unroll") for (int i = 1; i < kSeg - 1; i++) y[i] =                    \
+        fmaf(a_mid, x[i], fmaf(m, x[i - 1], fmaf(m, x[i + 1], K[i])));             \
+    y[kSeg - 1] = fmaf(a_end, x[kSeg - 1], fmaf(m, x[kSeg - 2], K[kSeg - 1]));     \
+    _Pragma("unroll") for (int i = 0; i < kSeg; i++) x[i] = y[i];                  \
+  }
+    int j = 1;
+    for (; j + 4 <= steps; j += 4) {
+      CKT_FAST_STEP CKT_FAST_STEP CKT_FAST_STEP CKT_FAST_STEP
+    }
+    for (; j < steps; j++) CKT_FAST_STEP
+#undef CKT_FAST_STEP
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) V[i] = fmaf(b, x[i] - x[i + 1], V[i]);
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) I[i] = x[i];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// calc_new_currents over SoA circuits.
+
+// Global-address-space views of the SoA arrays: loads and stores use
+// global_* instructions with a wave-uniform base, and the lane index is made
+// opaque after the step loop so that the 19 per-lane store addresses are
+// recomputed there instead of being held in VGPRs across the loop.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+template <int POLICY>
+__device__ __forceinline__ void cnc_one(const DevCircuit& c, uint32_t w, int steps) {
+  const size_t ws = c.ws;
+  gfloat* cur = (gfloat*)c.cur;
+  gfloat* vol = (gfloat*)c.vol;
+  float I[kSeg], V[kSeg - 1];
+#pragma unroll
+  for (int s = 0; s < kSeg; s++) I[s] = cur[s * ws + w];
+#pragma unroll
+  for (int s = 0; s < kSeg - 1; s++) V[s] = vol[s * ws + w];
+  const float L = ((gfloat*)c.ind)[w], R = ((gfloat*)c.res)[w], C = ((gfloat*)c.wcap)[w];
+  const float vin = ((gfloat*)c.voltage)[((gcu32*)c.in_ptr)[w]];
+  const float vout = ((gfloat*)c.voltage)[((gcu32*)c.out_ptr)[w]];
+  Relax<POLICY>::run(I, V, vin, vout, L, R, C, steps);
+  asm volatile("" : "+v"(w));
+#pragma unroll
+  for (int s = 0; s < kSeg; s++) cur[s * ws + w] = I[s];
+#pragma unroll
+  for (int s = 0; s < kSeg - 1; s++) vol[s * ws + w] = V[s];
+}
+
+template <int POLICY, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void cnc_kernel(const DevCircuit* __restrict__ circuits,
+                                                    const Tile* __restrict__ tiles, int steps) {
+  const Tile t = tiles[blockIdx.x];
+  if (threadIdx.x >= t.count) return;
+  cnc_one<POLICY>(circuits[t.circuit], t.first + threadIdx.x, steps);
+}
+
+// Persistent form: a fixed grid pulls tiles from a queue counter.  The last
+// block to finish resets the two counters, so consecutive launches on one
+// stream reuse them without a separate clear.
+template <int POLICY, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void cnc_persistent_kernel(const DevCircuit* __restrict__ circuits,
+                                                               const Tile* __restrict__ tiles,
+                                                               uint32_t ntiles, uint32_t* counters,
+                                                               int steps) {
+  __shared__ uint32_t s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&counters[0], 1u);
+    __syncthreads();
+    const uint32_t ti = s_tile;
+    __syncthreads();
+    if (ti >= ntiles) break;
+    const Tile t = tiles[ti];
+    if (threadIdx.x < t.count) cnc_one<POLICY>(circuits[t.circuit], t.first + threadIdx.x, steps);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(&counters[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&counters[0], 0u);
+      atomicExch(&counters[1], 0u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// distribute_charge.
+
+// One lane per wire, float atomics into the node charges.
+__global__ __launch_bounds__(256) void dsc_atomic_kernel(const DevCircuit* __restrict__ circuits,
+                                                         const Tile* __restrict__ tiles) {
+  const Tile t = tiles[blockIdx.x];
+  if (threadIdx.x >= t.count) return;
+  const DevCircuit& c = circuits[t.circuit];
+  const uint32_t w = t.first + threadIdx.x;
+  const float dt = CKT_DT_F;
+  const float q_in = -dt * c.cur[w];
+  const float q_out = dt * c.cur[(size_t)(kSeg - 1) * c.ws + w];
+  unsafeAtomicAdd(&c.charge[c.in_ptr[w]], q_in);
+  unsafeAtomicAdd(&c.charge[c.out_ptr[w]], q_out);
+}
+
+// One workgroup per piece: contributions to the piece's own node range are
+// summed in LDS, everything else goes straight to HBM with float atomics.
+// Private nodes receive contributions only from their own piece's wires, so
+// they are flushed with a plain read-add-write; shared nodes with atomics.
+__global__ __launch_bounds__(1024) void dsc_lds_kernel(const DevCircuit* __restrict__ circuits,
+                                                       const PieceTile* __restrict__ ptiles) {
+  extern __shared__ float acc[];
+  const PieceTile pt = ptiles[blockIdx.x];
+  const DevCircuit& c = circuits[pt.circuit];
+  const ckt_piece_t p = c.pieces[pt.piece];
+  for (uint32_t i = threadIdx.x; i < p.num_nodes; i += blockDim.x) acc[i] = 0.f;
+  __syncthreads();
+  const float dt = CKT_DT_F;
+  const size_t off9 = (size_t)(kSeg - 1) * c.ws;
+  for (uint32_t k = threadIdx.x; k < p.num_wires; k += blockDim.x) {
+    const uint32_t w = p.first_wire + k;
+    const float q_in = -dt * c.cur[w];
+    const float q_out = dt * c.cur[off9 + w];
+    const uint32_t a = c.in_ptr[w], b = c.out_ptr[w];
+    const uint32_t la = a - p.first_node, lb = b - p.first_node;
+    if (la < p.num_nodes) atomicAdd(&acc[la], q_in);
+    else unsafeAtomicAdd(&c.charge[a], q_in);
+    if (lb < p.num_nodes) atomicAdd(&acc[lb], q_out);
+    else unsafeAtomicAdd(&c.charge[b], q_out);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < p.num_nodes; i += blockDim.x) {
+    const uint32_t n = p.first_node + i;
+    const float v = acc[i];
+    if (c.locator[n]) c.charge[n] = c.charge[n] + v;
+    else if (v != 0.f) unsafeAtomicAdd(&c.charge[n], v);
+  }
+}
+
+// One lane per node: the node's contributions in (wire, in-before-out) order,
+// added one at a time to its charge -- the order in which the reference's
+// CPU tasks apply them when run one piece after another.
+template <bool KAHAN>
+__global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restrict__ circuits,
+                                                      const Tile* __restrict__ tiles) {
+  const Tile t = tiles[blockIdx.x];
+  if (threadIdx.x >= t.count) return;
+  const DevCircuit& c = circuits[t.circuit];
+  const uint32_t n = t.first + threadIdx.x;
+  const float dt = CKT_DT_F;
+  const float* cur0 = c.cur;
+  const float* cur9 = c.cur + (size_t)(kSeg - 1) * c.ws;
+  const uint32_t k0 = c.csr_row[n], k1 = c.csr_row[n + 1];
+  if (!KAHAN || c.locator[n]) {
+    float acc = c.charge[n];
+    for (uint32_t k = k0; k < k1; k++) {
+      const uint32_t e = c.csr_ent[k];
+      const uint32_t w = e >> 1;
+      const float v = (e & 1u) ? dt * cur9[w] : -dt * cur0[w];
+      acc += v;
+    }
+    c.charge[n] = acc;
+  } else {
+    double s = (double)c.charge[n], comp = 0.0;
+    for (uint32_t k = k0; k < k1; k++) {
+      const uint32_t e = c.csr_ent[k];
+      const uint32_t w = e >> 1;
+      const float v = (e & 1u) ? dt * cur9[w] : -dt * cur0[w];
+      const double y = (double)v - comp;
+      const double tt = s + y;
+      comp = (tt - s) - y;
+      s = tt;
+    }
+    c.charge[n] = (float)s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update_voltages (circuit_gpu.cu:330-345; circuit.cc:1299-1302).
+
+__global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__ circuits,
+                                                  const Tile* __restrict__ tiles) {
+  const Tile t = tiles[blockIdx.x];
+  if (threadIdx.x >= t.count) return;
+  const DevCircuit& c = circuits[t.circuit];
+  const uint32_t n = t.first + threadIdx.x;
+  const float q = c.charge[n], C = c.ncap[n], lk = c.leak[n];
+  float v = c.voltage[n];
+  v = v + q / C;
+  v = v * (1.f - lk);
+  c.voltage[n] = v;
+  c.charge[n] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Wire invariant checker (generalises sanity_check_positions,
+// circuit_gpu.cu:151-184): location codes valid, in-node inside the piece,
+// out-node inside the circuit, and location consistent with the locator and
+// the piece's node range.
+
+__global__ __launch_bounds__(256) void sanity_kernel(const DevCircuit* __restrict__ circuits,
+                                                     const Tile* __restrict__ tiles,
+                                                     unsigned long long* bad) {
+  const Tile t = tiles[blockIdx.x];
+  bool ok = true;
+  if (threadIdx.x < t.count) {
+    const DevCircuit& c = circuits[t.circuit];
+    const ckt_piece_t p = c.pieces[t.piece];
+    const uint32_t w = t.first + threadIdx.x;
+    const uint32_t a = c.in_ptr[w], b = c.out_ptr[w];
+    const uint32_t la = c.in_loc[w], lb = c.out_loc[w];
+    ok = la <= 2u && lb <= 2u && (a - p.first_node) < p.num_nodes && b < c.num_nodes;
+    if (ok) {
+      const uint32_t ends[2] = {a, b}, locs[2] = {la, lb};
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const bool own = (ends[e] - p.first_node) < p.num_nodes;
+        const bool priv = c.locator[ends[e]] != 0;
+        if (locs[e] == CKT_PRIVATE_PTR) ok = ok && own && priv;
+        else if (locs[e] == CKT_SHARED_PTR) ok = ok && own && !priv;
+        else ok = ok && !own && !priv;
+      }
+    }
+  }
+  const unsigned long long m = __ballot(!ok);
+  if ((threadIdx.x & 63u) == 0 && m) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+__global__ void nop_kernel() {}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Accessor-form kernels over the reference AoS layouts (absolute indexing,
+// GPU_Accessor::ref / AccessorGPUReductionFold, lowlevel.h:749-881).
+
+__device__ __forceinline__ const ckt_node_t* pick_instance(int32_t loc, const ckt_node_t* pvt,
+                                                           const ckt_node_t* owned,
+                                                           const ckt_node_t* ghost) {
+  switch (loc) { // get_node, circuit_gpu.cu:40-59
+    case CKT_PRIVATE_PTR: return pvt;
+    case CKT_SHARED_PTR: return owned;
+    case CKT_GHOST_PTR: return ghost;
+    default: return nullptr;
+  }
+}
+
+template <int POLICY>
+__global__ __launch_bounds__(256) void cnc_aos_kernel(uint32_t first, uint32_t num_wires,
+                                                      ckt_wire_t* __restrict__ wires,
+                                                      const ckt_node_t* pvt, const ckt_node_t* owned,
+                                                      const ckt_node_t* ghost, int steps,
+                                                      unsigned long long* bad) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= num_wires) return;
+  ckt_wire_t& wire = wires[first + tid];
+  const ckt_node_t* in_base = pick_instance(wire.in_loc, pvt, owned, ghost);
+  const ckt_node_t* out_base = pick_instance(wire.out_loc, pvt, owned, ghost);
+  if (!in_base || !out_base) {
+    atomicAdd(bad, 1ull);
+    return;
+  }
+  float I[kSeg], V[kSeg - 1];
+#pragma unroll
+  for (int s = 0; s < kSeg; s++) I[s] = wire.current[s];
+#pragma unroll
+  for (int s = 0; s < kSeg - 1; s++) V[s] = wire.voltage[s];
+  const float vin = in_base[wire.in_ptr].voltage, vout = out_base[wire.out_ptr].voltage;
+  Relax<POLICY>::run(I, V, vin, vout, wire.inductance, wire.resistance, wire.capacitance, steps);
+#pragma unroll
+  for (int s = 0; s < kSeg; s++) wire.current[s] = I[s];
+#pragma unroll
+  for (int s = 0; s < kSeg - 1; s++) wire.voltage[s] = V[s];
+}
+
+__device__ __forceinline__ bool reduce_local(int32_t loc, uint32_t ptr, float v, ckt_node_t* pvt,
+                                             float* owned_fold, float* ghost_fold) {
+  switch (loc) { // reduce_local, circuit_gpu.cu:219-243
+    case CKT_PRIVATE_PTR: unsafeAtomicAdd(&pvt[ptr].charge, v); return true;
+    case CKT_SHARED_PTR: unsafeAtomicAdd(&owned_fold[ptr], v); return true;
+    case CKT_GHOST_PTR: unsafeAtomicAdd(&ghost_fold[ptr], v); return true;
+    default: return false;
+  }
+}
+
+__global__ __launch_bounds__(256) void dsc_aos_kernel(uint32_t first, uint32_t num_wires,
+                                                      const ckt_wire_t* __restrict__ wires,
+                                                      ckt_node_t* pvt, float* owned_fold,
+                                                      float* ghost_fold, unsigned long long* bad) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= num_wires) return;
+  const ckt_wire_t& wire = wires[first + tid];
+  const float dt = CKT_DT_F;
+  bool ok = reduce_local(wire.in_loc, wire.in_ptr, -dt * wire.current[0], pvt, owned_fold, ghost_fold);
+  ok = reduce_local(wire.out_loc, wire.out_ptr, dt * wire.current[kSeg - 1], pvt, owned_fold,
+                    ghost_fold) && ok;
+  if (!ok) atomicAdd(bad, 1ull);
+}
+
+__global__ __launch_bounds__(256) void upv_aos_kernel(uint32_t first, uint32_t num_nodes,
+                                                      ckt_node_t* pvt, ckt_node_t* owned,
+                                                      const int32_t* __restrict__ locator) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= num_nodes) return;
+  const uint32_t i = first + tid;
+  ckt_node_t& nd = locator[i] ? pvt[i] : owned[i];
+  float v = nd.voltage;
+  v = v + nd.charge / nd.capacitance;
+  v = v * (1.f - nd.leakage);
+  nd.voltage = v;
+  nd.charge = 0.f;
+}
+
+__global__ __launch_bounds__(256) void sanity_aos_kernel(ckt_piece_t p, const ckt_wire_t* wires,
+                                                         uint32_t num_nodes_total,
+                                                         unsigned long long* bad) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = true;
+  if (tid < p.num_wires) {
+    const ckt_wire_t& wire = wires[p.first_wire + tid];
+    ok = (uint32_t)wire.in_loc <= 2u && (uint32_t)wire.out_loc <= 2u &&
+         (wire.in_ptr - p.first_node) < p.num_nodes && wire.out_ptr < num_nodes_total;
+  }
+  const unsigned long long m = __ballot(!ok);
+  if ((threadIdx.x & 63u) == 0 && m) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+__global__ __launch_bounds__(256) void fold_init_kernel(float* fold, uint32_t first, uint32_t count) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < count) fold[first + tid] = 0.0f; // AccumulateCharge::identity, circuit.cc:30
+}
+
+__global__ __launch_bounds__(256) void fold_apply_kernel(ckt_node_t* owner, const float* fold,
+                                                         uint32_t first, uint32_t count) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < count) owner[first + tid].charge += fold[first + tid]; // apply<true>, circuit.cc:32-36
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst,
+                                                   const uint4* __restrict__ src, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+static inline uint32_t blocks_for(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
+template <int POLICY>
+static hipError_t cnc_dispatch(const DevCircuit* c, const Tile* t, uint32_t n, int block, int steps,
+                               hipStream_t s) {
+  switch (block) {
+    case 64: hipLaunchKernelGGL((cnc_kernel<POLICY, 64>), dim3(n), dim3(64), 0, s, c, t, steps); break;
+    case 128: hipLaunchKernelGGL((cnc_kernel<POLICY, 128>), dim3(n), dim3(128), 0, s, c, t, steps); break;
+    case 256: hipLaunchKernelGGL((cnc_kernel<POLICY, 256>), dim3(n), dim3(256), 0, s, c, t, steps); break;
+    case 512: hipLaunchKernelGGL((cnc_kernel<POLICY, 512>), dim3(n), dim3(512), 0, s, c, t, steps); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cnc(int policy, const DevCircuit* c, const Tile* t, uint32_t n, int block,
+                      int steps, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  switch (policy) {
+    case CKT_POLICY_EXACT: return cnc_dispatch<CKT_POLICY_EXACT>(c, t, n, block, steps, s);
+    case CKT_POLICY_RECIP: return cnc_dispatch<CKT_POLICY_RECIP>(c, t, n, block, steps, s);
+    case CKT_POLICY_FAST: return cnc_dispatch<CKT_POLICY_FAST>(c, t, n, block, steps, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int POLICY>
+static hipError_t cncp_dispatch(const DevCircuit* c, const Tile* t, uint32_t n, int block, int grid,
+                                uint32_t* ctr, int steps, hipStream_t s) {
+  switch (block) {
+    case 64: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 64>), dim3(grid), dim3(64), 0, s, c, t, n, ctr, steps); break;
+    case 128: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 128>), dim3(grid), dim3(128), 0, s, c, t, n, ctr, steps); break;
+    case 256: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 256>), dim3(grid), dim3(256), 0, s, c, t, n, ctr, steps); break;
+    case 512: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 512>), dim3(grid), dim3(512), 0, s, c, t, n, ctr, steps); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cnc_persistent(int policy, const DevCircuit* c, const Tile* t, uint32_t n,
+                                 int block, int grid, uint32_t* ctr, int steps, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (grid < 1) return hipErrorInvalidValue;
+  switch (policy) {
+    case CKT_POLICY_EXACT: return cncp_dispatch<CKT_POLICY_EXACT>(c, t, n, block, grid, ctr, steps, s);
+    case CKT_POLICY_RECIP: return cncp_dispatch<CKT_POLICY_RECIP>(c, t, n, block, grid, ctr, steps, s);
+    case CKT_POLICY_FAST: return cncp_dispatch<CKT_POLICY_FAST>(c, t, n, block, grid, ctr, steps, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_dsc_atomic(const DevCircuit* c, const Tile* t, uint32_t n, int block,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (block != 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dsc_atomic_kernel, dim3(n), dim3(256), 0, s, c, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_dsc_lds(const DevCircuit* c, const PieceTile* pt, uint32_t n, size_t lds_bytes,
+                          hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  static bool attr_set = false;
+  if (!attr_set) {
+    // Allows dynamic LDS beyond the default limit where the runtime needs it.
+    (void)hipFuncSetAttribute((const void*)dsc_lds_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(dsc_lds_kernel, dim3(n), dim3(1024), lds_bytes, s, c, pt);
+  return hipGetLastError();
+}
+
+hipError_t launch_dsc_csr(int kahan, const DevCircuit* c, const Tile* t, uint32_t n, int block,
+                          hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (block != 256) return hipErrorInvalidValue;
+  if (kahan) hipLaunchKernelGGL(dsc_csr_kernel<true>, dim3(n), dim3(256), 0, s, c, t);
+  else hipLaunchKernelGGL(dsc_csr_kernel<false>, dim3(n), dim3(256), 0, s, c, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_upv(const DevCircuit* c, const Tile* t, uint32_t n, int block, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (block != 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(upv_kernel, dim3(n), dim3(256), 0, s, c, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_sanity(const DevCircuit* c, const Tile* t, uint32_t n, int block,
+                         unsigned long long* bad, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (block != 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sanity_kernel, dim3(n), dim3(256), 0, s, c, t, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_nop(hipStream_t s) {
+  hipLaunchKernelGGL(nop_kernel, dim3(1), dim3(64), 0, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cnc_aos(int policy, uint32_t first, uint32_t num, ckt_wire_t* wires,
+                          const ckt_node_t* pvt, const ckt_node_t* owned, const ckt_node_t* ghost,
+                          int steps, unsigned long long* bad, hipStream_t s) {
+  if (num == 0) return hipSuccess;
+  const dim3 g(blocks_for(num, 256)), b(256);
+  switch (policy) {
+    case CKT_POLICY_EXACT:
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_EXACT>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      break;
+    case CKT_POLICY_RECIP:
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_RECIP>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      break;
+    case CKT_POLICY_FAST:
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_FAST>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dsc_aos(uint32_t first, uint32_t num, const ckt_wire_t* wires, ckt_node_t* pvt,
+                          float* owned_fold, float* ghost_fold, unsigned long long* bad,
+                          hipStream_t s) {
+  if (num == 0) return hipSuccess;
+  hipLaunchKernelGGL(dsc_aos_kernel, dim3(blocks_for(num, 256)), dim3(256), 0, s, first, num, wires,
+                     pvt, owned_fold, ghost_fold, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_upv_aos(uint32_t first, uint32_t num, ckt_node_t* pvt, ckt_node_t* owned,
+                          const int32_t* locator, hipStream_t s) {
+  if (num == 0) return hipSuccess;
+  hipLaunchKernelGGL(upv_aos_kernel, dim3(blocks_for(num, 256)), dim3(256), 0, s, first, num, pvt,
+                     owned, locator);
+  return hipGetLastError();
+}
+
+hipError_t launch_sanity_aos(const ckt_piece_t p, const ckt_wire_t* wires, uint32_t num_nodes_total,
+                             unsigned long long* bad, hipStream_t s) {
+  if (p.num_wires == 0) return hipSuccess;
+  hipLaunchKernelGGL(sanity_aos_kernel, dim3(blocks_for(p.num_wires, 256)), dim3(256), 0, s, p,
+                     wires, num_nodes_total, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_init(float* fold, uint32_t first, uint32_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(fold_init_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, fold, first,
+                     count);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_apply(ckt_node_t* owner, const float* fold, uint32_t first, uint32_t count,
+                             hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(fold_apply_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, owner, fold,
+                     first, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return hipErrorInvalidValue;
+  const size_t n16 = bytes / 16;
+  uint32_t grid = blocks_for(n16, 256);
+  if (grid > 256u * 64u) grid = 256u * 64u;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
+  return hipGetLastError();
+}
+
+}  // namespace ckt

@@ -1,0 +1,69 @@
+"""Full-size (C2: 64 pieces x 16384 nodes x 65536 wires, ~4.2M wires) checks
+through size-independent properties.
+
+  * the generator is bit-identical to the oracle at this size;
+  * one EXACT loop: a random sample of wires matches the oracle's relaxation
+    bit for bit (the relaxation is per wire, so a sample is a full check of
+    those wires);
+  * node voltages equal the float64 scatter-add of the GPU's own currents
+    followed by the update (checks distribute_charge + update_voltages at
+    full size, any reduction order);
+  * CSR and ATOMIC reductions agree to 1e-5, and the reference profile gives
+    the oracle's NaN pattern on the sample.
+"""
+import numpy as np
+import pytest
+
+from helpers import TOL, assert_bit_identical, assert_same_class, max_rel_err
+
+pytestmark = pytest.mark.gpu
+
+C2 = (64, 16384, 65536, 95, 12345)
+
+
+@pytest.mark.parametrize("profile", [1, 0], ids=["stable", "reference"])
+def test_c2_one_loop_properties(C, O, profile):
+    P, npp, wpp, pct, seed = C2
+    host = C.generate(C.GenParams(P, npp, wpp, pct, seed, profile))
+    ref = O.generate(P, npp, wpp, pct, seed, profile)
+    assert host.wires.tobytes() == ref.wires.tobytes()
+    assert host.nodes.tobytes() == ref.nodes.tobytes()
+    del host
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(P, npp, wpp, pct, seed, profile))
+        ctx.run_loops(1, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+        nodes, wires = ctx.store()
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(len(ref.wires), 1500, replace=False))
+    for k in sample:
+        O.lib().orc_calc_new_currents_f32(0, O.STEPS, O._p(ref.wires), O._p(ref.nodes),
+                                          O._p(ref.nodes), O._p(ref.nodes), int(k), 1)
+    assert_bit_identical(wires["current"][sample], ref.wires["current"][sample], "sampled currents")
+    assert_bit_identical(wires["voltage"][sample], ref.wires["voltage"][sample], "sampled voltages")
+    # distribute_charge + update_voltages from the GPU's currents, in float64
+    dt = np.float32(1e-6)
+    q = np.zeros(len(nodes))
+    np.add.at(q, wires["in_ptr"], (-dt * wires["current"][:, 0]).astype(np.float64))
+    np.add.at(q, wires["out_ptr"], (dt * wires["current"][:, 9]).astype(np.float64))
+    v0 = ref.nodes["voltage"].astype(np.float64)
+    cap = ref.nodes["capacitance"].astype(np.float64)
+    leak = ref.nodes["leakage"].astype(np.float64)
+    want = (v0 + q / cap) * (1.0 - leak)
+    if profile == 1:
+        assert np.isfinite(nodes["voltage"]).all()
+        assert max_rel_err(nodes["voltage"], want) <= TOL
+        assert not nodes["charge"].any()
+    else:
+        assert_same_class(nodes["voltage"], want.astype(np.float32), "node voltage class")
+
+
+def test_c2_atomic_and_csr_agree(C):
+    P, npp, wpp, pct, seed = C2
+    out = {}
+    for mode in (C.REDUCE_ATOMIC, C.REDUCE_CSR):
+        with C.Context(0) as ctx:
+            ctx.generate(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE))
+            ctx.set_option(C.OPT_STEPS, 200)
+            ctx.run_loops(2, policy=C.POLICY_FAST, mode=mode)
+            out[mode] = ctx.store()[0]
+    assert max_rel_err(out[C.REDUCE_ATOMIC]["voltage"], out[C.REDUCE_CSR]["voltage"]) <= TOL

@@ -1,0 +1,265 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the oracle.
+
+Sizes: C1 (4 pieces x 1024 nodes x 4096 wires, 95% in-piece, seed 12345, 5
+loops of 10000 inner steps) and its pct-50 variant, in the reference profile
+(values diverge to inf/NaN, parity by class and bit pattern) and the stable
+profile (finite values).  EXACT + CSR must be bit-identical to the oracle;
+EXACT + KAHAN64 bit-identical to the oracle's Kahan restatement; the other
+policy/mode combinations within 1e-5 relative error on node voltages.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import (TOL, assert_bit_identical, assert_same_class, max_rel_err,
+                     max_wire_rel_err)
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(P=4, npp=1024, wpp=4096, pct=95, seed=12345)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(C, P, npp, wpp, pct, seed, profile, loops, policy, mode, **opts):
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(P, npp, wpp, pct, seed, profile))
+        for k, v in opts.items():
+            ctx.set_option(getattr(C, k), v)
+        ctx.run_loops(loops, policy=policy, mode=mode)
+        return ctx.store()
+
+
+_oracle_cache = {}
+
+
+def run_oracle(O, P, npp, wpp, pct, seed, profile, loops, reduce=0):
+    key = (P, npp, wpp, pct, seed, profile, loops, reduce)
+    if key not in _oracle_cache:
+        c = O.generate(P, npp, wpp, pct, seed, profile)
+        O.run(c, loops, reduce=reduce)
+        _oracle_cache[key] = c
+    return _oracle_cache[key]
+
+
+@pytest.mark.parametrize("profile", [0, 1], ids=["reference", "stable"])
+def test_c1_exact_csr_bit_identical(C, O, profile):
+    say(f"[c1 exact csr] profile={profile}")
+    nodes, wires = run_gpu(C, **C1, profile=profile, loops=5, policy=C.POLICY_EXACT,
+                           mode=C.REDUCE_CSR)
+    ref = run_oracle(O, **C1, profile=profile, loops=5)
+    assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+    assert_bit_identical(nodes["charge"], ref.nodes["charge"], "node charge")
+    assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
+    assert_bit_identical(wires["voltage"], ref.wires["voltage"], "wire voltage")
+    say(f"  NaN fraction of node voltages: {np.isnan(nodes['voltage']).mean():.4f}")
+
+
+def test_c3_pct50_exact_csr_bit_identical(C, O):
+    cfg = dict(C1, pct=50)
+    nodes, wires = run_gpu(C, **cfg, profile=1, loops=5, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+    ref = run_oracle(O, **cfg, profile=1, loops=5)
+    assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+    assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
+
+
+def test_c1_kahan64_bit_identical(C, O):
+    nodes, wires = run_gpu(C, **C1, profile=1, loops=5, policy=C.POLICY_EXACT,
+                           mode=C.REDUCE_CSR_KAHAN64)
+    ref = run_oracle(O, **C1, profile=1, loops=5, reduce=1)
+    assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+    assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2], ids=["exact", "recip", "fast"])
+@pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "csr"])
+def test_c1_stable_policies_within_tolerance(C, O, policy, mode):
+    nodes, wires = run_gpu(C, **C1, profile=1, loops=5, policy=policy, mode=mode)
+    ref = run_oracle(O, **C1, profile=1, loops=5)
+    e_v = max_rel_err(nodes["voltage"], ref.nodes["voltage"])
+    e_i = max_wire_rel_err(wires["current"], ref.wires["current"])
+    e_w = max_wire_rel_err(wires["voltage"], ref.wires["voltage"])
+    say(f"[c1 stable policy={policy} mode={mode}] node V {e_v:.2e} wire I {e_i:.2e} wire V {e_w:.2e}")
+    assert np.isfinite(nodes["voltage"]).all()
+    assert e_v <= TOL
+    assert e_i <= TOL
+    assert e_w <= TOL
+
+
+@pytest.mark.parametrize("policy", [1, 2], ids=["recip", "fast"])
+def test_c1_reference_profile_fast_policies_match_nan_classes(C, O, policy):
+    nodes, wires = run_gpu(C, **C1, profile=0, loops=5, policy=policy, mode=C.REDUCE_ATOMIC)
+    ref = run_oracle(O, **C1, profile=0, loops=5)
+    assert_same_class(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+    assert_same_class(wires["current"], ref.wires["current"], "wire current")
+    fin = np.isfinite(ref.nodes["voltage"])
+    assert max_rel_err(nodes["voltage"][fin], ref.nodes["voltage"][fin]) <= TOL
+
+
+def test_run_loop_modes_agree(C):
+    results = []
+    for sync, graph, per_piece in [(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0), (0, 0, 1)]:
+        with C.Context(0) as ctx:
+            ctx.generate(C.GenParams(4, 256, 1024, 90, 3, C.PROFILE_STABLE))
+            rep = ctx.run_loops(3, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR, sync=sync,
+                                use_graph=bool(graph), per_piece=bool(per_piece))
+            assert rep["elapsed_s"] > 0 and rep["gflops"] > 0
+            results.append(ctx.store())
+    for nodes, wires in results[1:]:
+        assert nodes.tobytes() == results[0][0].tobytes()
+        assert wires.tobytes() == results[0][1].tobytes()
+
+
+def test_task_variant_table_and_nops(C, O):
+    params = C.GenParams(4, 128, 512, 80, 21, C.PROFILE_STABLE)
+    with C.Context(0) as ctx:
+        ctx.generate(params)
+        pieces = ctx.pieces()
+        for task in (C.CALC_NEW_CURRENTS_NOP, C.CALC_NEW_CURRENTS, C.DISTRIBUTE_CHARGE_NOP,
+                     C.DISTRIBUTE_CHARGE, C.UPDATE_VOLTAGES_NOP, C.UPDATE_VOLTAGES,
+                     C.SANITY_CHECK_WIRES):
+            ctx.execute_index_space(task, pieces, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+        nodes, wires = ctx.store()
+        with pytest.raises(C.CircuitError):
+            ctx.execute_index_space(C.REGION_MAIN, pieces)
+        with pytest.raises(C.CircuitError):
+            ctx.execute_index_space(99, pieces)
+    ref = O.generate(4, 128, 512, 80, 21, O.PROFILE_STABLE)
+    O.run(ref, 1)
+    assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+    assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
+
+
+def test_per_piece_calls_and_errors(C, O):
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(4, 128, 512, 80, 22, C.PROFILE_STABLE))
+        pieces = ctx.pieces()
+        # calc_new_currents one piece at a time, in reverse order
+        for p in pieces[::-1]:
+            ctx.calc_new_currents(p[None], C.POLICY_EXACT)
+        ctx.distribute_charge(pieces, C.REDUCE_CSR)
+        ctx.update_voltages(pieces)
+        nodes, wires = ctx.store()
+        # CSR needs the whole index space in one launch
+        with pytest.raises(C.CircuitError) as e:
+            ctx.distribute_charge(pieces[:2], C.REDUCE_CSR)
+        assert e.value.status == 8
+        bogus = pieces[:1].copy()
+        bogus["first_wire"] += 1
+        with pytest.raises(C.CircuitError) as e:
+            ctx.calc_new_currents(bogus)
+        assert e.value.status == 2
+        with pytest.raises(C.CircuitError):
+            ctx.calc_new_currents(pieces, policy=7)
+    ref = O.generate(4, 128, 512, 80, 22, O.PROFILE_STABLE)
+    O.run(ref, 1)
+    assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
+
+
+def test_one_inner_step_is_ohms_law_on_gpu(C):
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(2, 64, 256, 80, 5, C.PROFILE_STABLE))
+        ctx.set_option(C.OPT_STEPS, 1)
+        n0, w0 = ctx.store()
+        for policy in (C.POLICY_EXACT, C.POLICY_RECIP, C.POLICY_FAST):
+            ctx.generate(C.GenParams(2, 64, 256, 80, 5, C.PROFILE_STABLE))
+            ctx.calc_new_currents(None, policy)
+            _, w = ctx.store()
+            vin = n0["voltage"][w0["in_ptr"]]
+            vout = n0["voltage"][w0["out_ptr"]]
+            want0 = -vin / w0["resistance"]
+            want9 = vout / w0["resistance"]
+            if policy == C.POLICY_EXACT:
+                assert np.array_equal(w["current"][:, 0], want0)
+                assert np.array_equal(w["current"][:, 9], want9)
+            else:
+                assert np.allclose(w["current"][:, 0], want0, rtol=1e-6)
+                assert np.allclose(w["current"][:, 9], want9, rtol=1e-6)
+            assert not w["current"][:, 1:9].any()
+
+
+def test_accessor_form_dropins(C, O):
+    """The _gpu entry points on reference-layout instances (separate pvt,
+    owned and ghost instances, fold buffers, int locator)."""
+    c = O.generate(4, 256, 1024, 70, 31, O.PROFILE_STABLE)
+    N = len(c.nodes)
+    dw = C.DeviceArray(c.wires)
+    pvt, owned, ghost = (C.DeviceArray(c.nodes) for _ in range(3))
+    ref = c.copy()
+    for p in c.pieces:
+        C.calc_new_currents_gpu(p, dw, pvt, owned, ghost, flag=C.POLICY_EXACT)
+        O.calc_new_currents(ref, p)
+    assert_bit_identical(dw.download()["current"], ref.wires["current"], "wire current")
+    assert_bit_identical(dw.download()["voltage"], ref.wires["voltage"], "wire voltage")
+    # distribute_charge: private into pvt, shared/ghost into fold buffers
+    zeros = np.zeros(N, np.float32)
+    ofold, gfold = C.DeviceArray(zeros), C.DeviceArray(zeros)
+    C.fold_init_gpu(ofold, 0, N)
+    C.fold_init_gpu(gfold, 0, N)
+    want_pvt = ref.nodes.copy()
+    want_o, want_g = zeros.copy(), zeros.copy()
+    for p in c.pieces:
+        C.distribute_charge_gpu(p, dw, pvt, ofold, gfold)
+        O.distribute_charge_fold(ref.wires, want_pvt, want_o, want_g, int(p["first_wire"]),
+                                 int(p["num_wires"]))
+    assert max_rel_err(pvt.download()["charge"], want_pvt["charge"], floor=1e-12) <= TOL
+    assert max_rel_err(ofold.download(), want_o, floor=1e-12) <= TOL
+    assert max_rel_err(gfold.download(), want_g, floor=1e-12) <= TOL
+    # apply the fold buffers to the owner (shared) instance
+    C.fold_apply_gpu(owned, ofold, 0, N)
+    C.fold_apply_gpu(owned, gfold, 0, N)
+    # update_voltages from an exactly known pre-state
+    pre = ref.nodes.copy()
+    pre["charge"] = np.random.default_rng(0).uniform(-1e-3, 1e-3, N).astype(np.float32)
+    pvt.upload(pre)
+    owned.upload(pre)
+    locator = C.DeviceArray(c.locator)
+    want_p, want_s = pre.copy(), pre.copy()
+    for p in c.pieces:
+        C.update_voltages_gpu(p, pvt, owned, locator)
+        O.update_voltages(ref, p, pvt=want_p, shr=want_s)
+    assert pvt.download().tobytes() == want_p.tobytes()
+    assert owned.download().tobytes() == want_s.tobytes()
+    # sanity check: clean wires pass; an in-node moved out of its piece fails
+    assert C.sanity_check_wires_gpu(c.pieces[0], dw, N) == 0
+    bad = c.wires.copy()
+    bad["in_ptr"][5] = int(c.pieces[1]["first_node"])
+    bad["out_loc"][7] = 3
+    dw.upload(bad)
+    assert C.sanity_check_wires_gpu(c.pieces[0], dw, N) == 2
+    # an invalid PointerLocation is reported, not followed
+    with pytest.raises(C.CircuitError):
+        C.calc_new_currents_gpu(c.pieces[0], dw, pvt, owned, ghost)
+
+
+def test_sanity_check_on_context(C):
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(**{"num_pieces": 8, "nodes_per_piece": 512,
+                                    "wires_per_piece": 2048, "pct_wire_in_piece": 60}))
+        assert ctx.sanity_check_wires() == 0
+
+
+def test_load_validation(C):
+    host = C.generate(C.GenParams(2, 32, 64, 50, 1))
+    with C.Context(0) as ctx:
+        bad = host.copy()
+        bad.wires["out_ptr"][0] = 10**6
+        with pytest.raises(C.CircuitError) as e:
+            ctx.load(bad)
+        assert e.value.status == 2
+        bad = host.copy()
+        bad.wires["in_loc"][3] = 5
+        with pytest.raises(C.CircuitError):
+            ctx.load(bad)
+        bad = host.copy()
+        # a private node of piece 1 reached from piece 0
+        priv1 = np.flatnonzero(bad.locator[32:] == 1)[0] + 32
+        bad.wires["out_ptr"][0] = priv1
+        with pytest.raises(C.CircuitError):
+            ctx.load(bad)
+        ctx.load(host)
+        assert ctx.sizes() == (64, 128, 2)

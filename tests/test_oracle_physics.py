@@ -1,0 +1,234 @@
+"""Pins the oracle's task bodies against what the mathematics fixes.
+
+calc_new_currents (circuit.cc:352-413, GPU seeding circuit_gpu.cu:107-112)
+  * one step from rest is Ohm's law per segment, and the internal voltages
+    then follow the capacitor law (exact in FP32);
+  * the relaxation's fixed point is the backward-Euler step of the RLC
+    ladder: KVL  L (I_i - I0_i)/dt + R I_i = v_{i+1} - v_i  and
+    KCL  C (v_k - V0_k)/dt = I_{k-1} - I_k, solved here with numpy;
+  * with the reference parameters the currents grow by -L/(dt R) per step
+    and overflow FP32 after about 8-10 steps (divergence signature);
+  * the CPU-literal seeding never reads the in-node voltage.
+distribute_charge (circuit.cc:415-443)
+  * charge conservation and a float64 scatter-add per node.
+update_voltages (circuit.cc:1290-1307)
+  * special cases q = 0 and leak = 0, charge cleared, locator dispatch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+f32 = np.float32
+DT32 = f32(1e-6)
+
+
+def _stable(O, P=2, npp=64, wpp=256, pct=80, seed=11):
+    return O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE)
+
+
+def test_one_step_from_rest_is_ohms_law(O):
+    c = _stable(O)
+    O.calc_new_currents(c, steps=1)
+    ref = _stable(O)
+    w, n = ref.wires, ref.nodes
+    vin = n["voltage"][w["in_ptr"]]
+    vout = n["voltage"][w["out_ptr"]]
+    R, C = w["resistance"], w["capacitance"]
+    I = c.wires["current"]
+    # segment 0 sees the in-node, segment 9 the out-node, the rest zero
+    assert np.array_equal(I[:, 0], (f32(0) - vin) / R)
+    assert np.array_equal(I[:, 9], (vout - f32(0)) / R)
+    assert not I[:, 1:9].any()
+    V = c.wires["voltage"]
+    assert np.array_equal(V[:, 0], (DT32 * (I[:, 0] - I[:, 1])) / C)
+    assert np.array_equal(V[:, 8], (DT32 * (I[:, 8] - I[:, 9])) / C)
+    assert not V[:, 1:8].any()
+
+
+def test_cpu_literal_seeding_ignores_in_node(O):
+    a = _stable(O)
+    b = _stable(O)
+    b.nodes["voltage"][:] += f32(0.5)  # changes every in/out node voltage
+    b.nodes["voltage"][b.wires["out_ptr"]] = a.nodes["voltage"][a.wires["out_ptr"]]
+    O.calc_new_currents(a, steps=3, seeding=O.SEED_CPU_LITERAL)
+    O.calc_new_currents(b, steps=3, seeding=O.SEED_CPU_LITERAL)
+    only_in = ~np.isin(a.wires["in_ptr"], a.wires["out_ptr"])
+    assert np.array_equal(a.wires["current"][only_in], b.wires["current"][only_in])
+    a2, b2 = _stable(O), _stable(O)
+    b2.nodes["voltage"][:] += f32(0.5)
+    O.calc_new_currents(a2, steps=3)
+    O.calc_new_currents(b2, steps=3)
+    assert not np.array_equal(a2.wires["current"][:, 0], b2.wires["current"][:, 0])
+
+
+def _backward_euler(L, R, C, I0, V0, vin, vout, dt):
+    """Solve KVL per segment and KCL per internal node for one time step."""
+    n_i, n_v = 10, 9
+    A = np.zeros((19, 19))
+    b = np.zeros(19)
+    for i in range(n_i):  # L(I_i - I0_i)/dt + R I_i - v_{i+1} + v_i = 0
+        A[i, i] = L / dt + R
+        b[i] = L / dt * I0[i]
+        if i + 1 <= 9:
+            A[i, n_i + i] -= 1.0  # v_{i+1}
+        else:
+            b[i] += vout
+        if i >= 1:
+            A[i, n_i + i - 1] += 1.0  # v_i
+        else:
+            b[i] -= vin
+    for k in range(1, 10):  # C(v_k - V0_k)/dt - I_{k-1} + I_k = 0
+        r = n_i + k - 1
+        A[r, r] = C / dt
+        A[r, k - 1] = -1.0
+        A[r, k] = 1.0
+        b[r] = C / dt * V0[k - 1]
+    x = np.linalg.solve(A, b)
+    return x[:n_i], x[n_i:]
+
+
+def test_fixed_point_is_backward_euler_rlc_step(O):
+    c0 = _stable(O, P=1, npp=64, wpp=96, pct=100, seed=5)
+    rng = np.random.default_rng(1)
+    c0.wires["current"][:] = rng.uniform(-0.5, 0.5, c0.wires["current"].shape).astype(f32)
+    c0.wires["voltage"][:] = rng.uniform(-1, 1, c0.wires["voltage"].shape).astype(f32)
+    c = c0.copy()
+    n64, w64 = O.promote(c)
+    O.calc_new_currents_f64(n64, w64, 0, len(w64))
+    O.calc_new_currents(c)  # FP32, 10000 steps
+    dt = float(DT32)
+    worst64 = worst32 = 0.0
+    for k in range(len(c0.wires)):
+        w0 = c0.wires[k]
+        vin = float(c0.nodes["voltage"][w0["in_ptr"]])
+        vout = float(c0.nodes["voltage"][w0["out_ptr"]])
+        I, V = _backward_euler(float(w0["inductance"]), float(w0["resistance"]),
+                               float(w0["capacitance"]), w0["current"].astype(float),
+                               w0["voltage"].astype(float), vin, vout, dt)
+        scale = max(np.abs(I).max(), 1e-30)
+        worst64 = max(worst64, np.abs(w64[k]["current"] - I).max() / scale)
+        worst32 = max(worst32, np.abs(c.wires[k]["current"] - I).max() / scale)
+        vs = max(np.abs(V).max(), 1e-30)
+        worst64 = max(worst64, np.abs(w64[k]["voltage"] - V).max() / vs)
+        worst32 = max(worst32, np.abs(c.wires[k]["voltage"] - V).max() / vs)
+    assert worst64 < 1e-9, worst64
+    assert worst32 < 1e-5, worst32
+
+
+def test_reference_parameters_diverge_with_gain_L_over_dt_R(O):
+    c = O.generate(1, 256, 200, 100, 12345, O.PROFILE_REFERENCE)
+    n64, w64a = O.promote(c)
+    _, w64b = O.promote(c)
+    O.calc_new_currents_f64(n64, w64a, 0, len(w64a), steps=6)
+    O.calc_new_currents_f64(n64, w64b, 0, len(w64b), steps=7)
+    g = c.wires["inductance"].astype(float) / (float(DT32) * c.wires["resistance"].astype(float))
+    ratio = w64b["current"] / w64a["current"]
+    # From rest, step 1 excites only the end segments; afterwards each step
+    # multiplies by T = -g + d(2 - shift).  A segment k places from the nearer
+    # end therefore carries C(j-1, k) (-g)^(j-1-k) d^k, and from step j=6 to 7
+    # grows by -g * 6/(6-k) (up to O(d/g) and O(1/g) terms).
+    for k in (0, 1, 2):
+        want = -g * 6.0 / (6 - k)
+        assert np.allclose(ratio[:, k], want, rtol=1e-3)
+        assert np.allclose(ratio[:, 9 - k], want, rtol=1e-3)
+    # FP32: finite after 7 steps, overflowed and NaN well before 25
+    a = c.copy()
+    O.calc_new_currents(a, steps=7)
+    assert np.isfinite(a.wires["current"]).all()
+    b = c.copy()
+    O.calc_new_currents(b, steps=25)
+    assert np.isnan(b.wires["current"]).all()
+
+
+def test_distribute_charge_conserves_charge(O):
+    c = O.generate(4, 256, 1024, 70, 3, O.PROFILE_STABLE)
+    rng = np.random.default_rng(2)
+    c.wires["current"][:] = rng.uniform(-1, 1, c.wires["current"].shape).astype(f32)
+    for p in c.pieces:
+        O.distribute_charge(c, p)
+    w = c.wires
+    qin = (-DT32 * w["current"][:, 0]).astype(np.float64)
+    qout = (DT32 * w["current"][:, 9]).astype(np.float64)
+    exact = np.zeros(len(c.nodes))
+    np.add.at(exact, w["in_ptr"], qin)
+    np.add.at(exact, w["out_ptr"], qout)
+    absum = np.zeros(len(c.nodes))
+    np.add.at(absum, w["in_ptr"], np.abs(qin))
+    np.add.at(absum, w["out_ptr"], np.abs(qout))
+    err = np.abs(c.nodes["charge"].astype(np.float64) - exact)
+    assert np.all(err <= 2 * np.finfo(f32).eps * absum * 8 + 1e-30)
+    assert math.isclose(c.nodes["charge"].astype(np.float64).sum(), qin.sum() + qout.sum(),
+                        rel_tol=1e-4, abs_tol=1e-9)
+
+
+def test_kahan64_is_correctly_rounded_sum_on_shared_nodes(O):
+    c = O.generate(4, 128, 1024, 50, 9, O.PROFILE_STABLE)
+    rng = np.random.default_rng(3)
+    c.wires["current"][:] = rng.uniform(-1, 1, c.wires["current"].shape).astype(f32)
+    seq = c.copy()
+    for p in seq.pieces:
+        O.distribute_charge(seq, p)
+    O.distribute_charge_kahan64(c)
+    w = c.wires
+    exact = np.zeros(len(c.nodes))
+    vals = []
+    for n in range(len(c.nodes)):
+        vals.append([])
+    for k in range(len(w)):
+        vals[w["in_ptr"][k]].append(float(-DT32 * w["current"][k, 0]))
+        vals[w["out_ptr"][k]].append(float(DT32 * w["current"][k, 9]))
+    for n in range(len(c.nodes)):
+        exact[n] = math.fsum(vals[n])
+    shared = c.locator == 0
+    got = c.nodes["charge"][shared]
+    want = exact[shared].astype(f32)
+    ulp = np.spacing(np.abs(want))
+    assert np.all(np.abs(got - want) <= ulp)
+    assert np.array_equal(c.nodes["charge"][~shared], seq.nodes["charge"][~shared])
+
+
+def test_fold_buffers_match_direct_reduction(O):
+    c = O.generate(4, 128, 512, 60, 4, O.PROFILE_STABLE)
+    rng = np.random.default_rng(4)
+    c.wires["current"][:] = rng.uniform(-1, 1, c.wires["current"].shape).astype(f32)
+    direct = c.copy()
+    for p in direct.pieces:
+        O.distribute_charge(direct, p)
+    owner = c.nodes.copy()
+    N = len(owner)
+    for p in c.pieces:
+        shr = np.zeros(N, f32)
+        ghost = np.zeros(N, f32)
+        O.distribute_charge_fold(c.wires, owner, shr, ghost, int(p["first_wire"]),
+                                 int(p["num_wires"]))
+        O.fold_apply(owner, shr, 0, N)
+        O.fold_apply(owner, ghost, 0, N)
+    priv = c.locator == 1
+    assert np.array_equal(owner["charge"][priv], direct.nodes["charge"][priv])
+    assert np.allclose(owner["charge"], direct.nodes["charge"], rtol=1e-5, atol=1e-12)
+
+
+def test_update_voltages_special_cases(O):
+    c = O.generate(2, 512, 16, 90, 8)
+    n = c.nodes
+    rng = np.random.default_rng(5)
+    n["charge"][:] = rng.uniform(-1e-3, 1e-3, len(n)).astype(f32)
+    n["charge"][:100] = 0
+    n["leakage"][100:200] = 0
+    before = n.copy()
+    pvt = c.nodes.copy()
+    shr = c.nodes.copy()
+    for p in c.pieces:
+        O.update_voltages(c, p, pvt=pvt, shr=shr)
+    priv = c.locator == 1
+    got = np.where(priv, pvt["voltage"], shr["voltage"])
+    # q = 0: pure leak
+    assert np.array_equal(got[:100], before["voltage"][:100] * (f32(1) - before["leakage"][:100]))
+    # leak = 0: pure charge-in
+    i = slice(100, 200)
+    assert np.array_equal(got[i], before["voltage"][i] + before["charge"][i] / before["capacitance"][i])
+    # charge cleared in the instance that owns the node; the other untouched
+    assert not np.where(priv, pvt["charge"], shr["charge"]).any()
+    assert np.array_equal(np.where(priv, shr["voltage"], pvt["voltage"]),
+                          np.where(priv, before["voltage"], before["voltage"]))
