@@ -15,6 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libckt.so")
+DRIVER = os.path.join(HERE, "ckt_sim")
 BUILD_DIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
@@ -54,8 +55,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # Host driver (region_main replacement) linked against the library.
+    cmd = [hipcc(), "-O2", "-std=c++17", "-o", DRIVER + ".tmp", os.path.join(CSRC, "ckt_main.cpp"),
+           "-L" + HERE, "-lckt", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
 
 

@@ -535,6 +535,84 @@ class Batch:
         return best, ms.value
 
 
+@dataclass
+class Request:
+    gen: GenParams
+    num_loops: int
+    arrival_s: float
+
+
+class Server:
+    """Continuous batching of circuit simulations (replays a request trace
+    in real time; see ckt_server_run_trace)."""
+
+    def __init__(self, device=0, budget_bytes=0, max_active_wires=0, max_active=0,
+                 policy=POLICY_FAST, mode=REDUCE_ATOMIC, gen_threads=8):
+        self.h = ctypes.c_void_p()
+        cfg = _ServerConfig(budget_bytes, max_active_wires, max_active, policy, mode, gen_threads)
+        _check(lib().ckt_server_create(device, ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ckt_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_keep_state(self, keep=True):
+        _check(lib().ckt_server_set_keep_state(self.h, int(keep)))
+
+    def run_trace(self, requests):
+        n = len(requests)
+        reqs = (_Request * n)()
+        for i, r in enumerate(requests):
+            reqs[i] = _Request(r.gen.c(), r.num_loops, r.arrival_s)
+        res = (_RequestResult * n)()
+        rep = _ServerReport()
+        _check(lib().ckt_server_run_trace(self.h, reqs, n, res, ctypes.byref(rep)))
+        results = [{"status": x.status, "num_wires": x.num_wires, "num_loops": x.num_loops,
+                    "arrival_s": x.arrival_s, "admit_s": x.admit_s, "finish_s": x.finish_s,
+                    "checksum": x.final_voltage_checksum} for x in res]
+        report = {k: getattr(rep, k) for k, _ in _ServerReport._fields_}
+        return results, report
+
+    def fetch(self, index, num_nodes, num_wires):
+        nodes = np.zeros(num_nodes, NODE)
+        wires = np.zeros(num_wires, WIRE)
+        _check(lib().ckt_server_fetch(self.h, index, _p(nodes), _p(wires)))
+        return nodes, wires
+
+
+def poisson_trace(rate_per_s, duration_s, seed, wires_lo=2**12, wires_hi=2**20, loops_lo=1,
+                  loops_hi=20, pct=95, profile=PROFILE_REFERENCE, base_seed=1000):
+    """The served workload: Poisson arrivals; per request wires log-uniform in
+    [wires_lo, wires_hi], nodes = wires/4, pieces = ceil(nodes/4096), uniform
+    pieces (remainders dropped), loops uniform in [loops_lo, loops_hi]."""
+    rng = np.random.default_rng(seed)
+    t, out, i = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate_per_s)
+        if t > duration_s:
+            break
+        W = int(np.exp(rng.uniform(np.log(wires_lo), np.log(wires_hi))))
+        out.append(Request(shape_for_wires(W, pct, base_seed + i, profile),
+                           int(rng.integers(loops_lo, loops_hi + 1)), t))
+        i += 1
+    return out
+
+
+def shape_for_wires(W, pct=95, seed=0, profile=PROFILE_REFERENCE, nodes_per_piece_max=4096):
+    """Uniform-piece circuit of about W wires: nodes = W/4, pieces =
+    ceil(nodes/4096), nodes/piece and wires/piece rounded down."""
+    nodes = max(1, W // 4)
+    pieces = -(-nodes // nodes_per_piece_max)
+    return GenParams(pieces, max(1, nodes // pieces), max(1, W // pieces), pct, seed, profile)
+
+
 def footprint_bytes(num_nodes, num_wires, num_pieces):
     b = ctypes.c_uint64()
     _check(lib().ckt_footprint_bytes(num_nodes, num_wires, num_pieces, ctypes.byref(b)))
