@@ -44,6 +44,19 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the per-policy and per-kernel detail runs")
+    ap.add_argument("--batched", action="store_true",
+                    help="batched workload: 512 circuits in shared launches vs sequential")
+    ap.add_argument("--served", action="store_true",
+                    help="served workload: Poisson request stream with continuous batching")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-cap-wires", type=int, default=8 << 20)
+    ap.add_argument("--compare-seconds", type=float, default=2.0,
+                    help="served: trace length of the batched-vs-sequential comparison")
+    ap.add_argument("--upv-sweep", action="store_true",
+                    help="update_voltages HBM bandwidth at growing node counts")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run only the plain copy kernel over BYTES (profiler calibration)")
     return ap.parse_args()
 
 
@@ -93,6 +106,143 @@ def cpu_baseline(C, params, seconds):
             "seconds": t}
 
 
+def batched_shapes(C, n=512, lo=2**10, hi=2**18, seed=4, profile=None):
+    """The batched workload: wires log-uniform in [lo, hi], nodes = wires/4,
+    pieces = ceil(nodes/4096), 95% in-piece, seeds 1000+i."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    prof = C.PROFILE_REFERENCE if profile is None else profile
+    return [C.shape_for_wires(int(np.exp(rng.uniform(np.log(lo), np.log(hi)))), 95, 1000 + i, prof)
+            for i in range(n)]
+
+
+def run_batched(C, args):
+    loops = 5
+    shapes = batched_shapes(C)
+    W_total = sum(s.num_pieces * s.wires_per_piece for s in shapes)
+    b = C.Batch(0)
+    t0 = time.perf_counter()
+    b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, os.cpu_count() or 1))
+    log(f"[batched] {len(shapes)} circuits, {W_total} wires, added in "
+        f"{time.perf_counter() - t0:.1f} s")
+    best, probe_ms = b.autotune(C.POLICY_FAST, probe_steps=500)
+    log(f"[batched] autotune: block={best.block_threads} persistent={best.persistent} "
+        f"waves/SIMD={best.waves_per_simd} ({probe_ms:.2f} ms per 500-step probe)")
+    best.policy, best.reduce_mode, best.use_graph = C.POLICY_FAST, C.REDUCE_ATOMIC, 1
+    b.set_timing(True)
+    rep = b.run(best)
+    b.close()
+    log(f"[batched] {rep}")
+    # Sequential plain launches: each circuit alone, (a) one launch per piece
+    # per task with a wait after each (how the reference runtime runs GPU
+    # point tasks), (b) one launch per task over all pieces, graph-captured.
+    seq_piece = seq_task = 0.0
+    ctx = C.Context(0)
+    for i, s in enumerate(shapes):
+        ctx.generate(s)
+        seq_piece += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+                                   per_piece=True)["elapsed_s"]
+        seq_task += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+                                  use_graph=True)["elapsed_s"]
+        if i % 64 == 63:
+            log(f"[batched] sequential: {i + 1} circuits")
+    ctx.close()
+    wu = W_total * loops
+    out = {"metric": "batched wire-updates/s (512 circuits, one launch per task)",
+           "value": rep["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
+           "dtype": "f32", "data": "synthetic (load_circuit generator, seeds 1000+i)",
+           "config": {"workload": "512 circuits, wires log-uniform in [2^10, 2^18], nodes = wires/4, "
+                                  "pieces = ceil(nodes/4096), 95% in-piece, 10000 inner steps, "
+                                  "5 loops each, FAST policy, ATOMIC reduction, hipGraph per loop",
+                      "total_wires": W_total,
+                      "autotuned": {"block_threads": best.block_threads,
+                                    "persistent": best.persistent,
+                                    "waves_per_simd": best.waves_per_simd}},
+           "batched_elapsed_s": rep["elapsed_s"],
+           "sequential_per_piece": {"elapsed_s": seq_piece, "wire_updates_per_s": wu / seq_piece},
+           "sequential_per_task_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
+           "speedup_vs_sequential_per_piece": seq_piece / rep["elapsed_s"],
+           "speedup_vs_sequential_per_task_graph": seq_task / rep["elapsed_s"]}
+    print(json.dumps(out))
+
+
+def run_served(C, args):
+    budget = 200 * 10**9
+    trace = C.poisson_trace(args.serve_rate, args.serve_seconds, seed=5)
+    offered = sum(r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops for r in trace)
+    log(f"[served] {len(trace)} requests over {args.serve_seconds} s, "
+        f"{offered:.3g} wire-updates offered ({offered / args.serve_seconds:.3g}/s)")
+    s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
+                 policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC, gen_threads=12)
+    res, rep = s.run_trace(trace)
+    s.close()
+    log(f"[served] {rep}")
+    # Continuous batching vs one request at a time on a short trace.
+    short = C.poisson_trace(args.serve_rate, args.compare_seconds, seed=6)
+    cmp = {}
+    for name, max_active in (("continuous", 0), ("sequential", 1)):
+        s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
+                     max_active=max_active, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+                     gen_threads=12)
+        _, r2 = s.run_trace(short)
+        s.close()
+        cmp[name] = {k: r2[k] for k in ("wire_updates_per_s", "elapsed_s", "latency_p50_s",
+                                        "latency_p99_s", "completed")}
+        log(f"[served] {name}: {cmp[name]}")
+    out = {"metric": "served wire-updates/s with p50/p99 request latency",
+           "value": rep["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
+           "dtype": "f32", "data": "synthetic (Poisson arrivals, load_circuit generator)",
+           "config": {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s; "
+                                  "wires log-uniform in [2^12, 2^20], nodes = wires/4, "
+                                  "pieces = ceil(nodes/4096), 95% in-piece, loops U[1,20], "
+                                  "FAST policy, ATOMIC reduction",
+                      "hbm_budget_bytes": budget, "max_active_wires": args.serve_cap_wires},
+           "latency_p50_s": rep["latency_p50_s"], "latency_p99_s": rep["latency_p99_s"],
+           "latency_mean_s": rep["latency_mean_s"], "latency_max_s": rep["latency_max_s"],
+           "offered_wire_updates_per_s": rep["offered_wire_updates_per_s"],
+           "completed": rep["completed"], "rejected": rep["rejected"],
+           "iterations": rep["iterations"], "max_resident": rep["max_resident"],
+           "elapsed_s": rep["elapsed_s"],
+           "compare_short_trace": {"seconds": args.compare_seconds, **cmp,
+                                   "speedup": cmp["continuous"]["wire_updates_per_s"]
+                                   / max(cmp["sequential"]["wire_updates_per_s"], 1e-30)}}
+    print(json.dumps(out))
+
+
+def run_upv_sweep(C, args):
+    rows = []
+    for P, npp in ((64, 16384), (256, 65536), (1024, 65536), (2048, 65536)):
+        N = P * npp
+        ctx = C.Context(0)
+        ctx.generate(C.GenParams(P, npp, 0, 95, 1, C.PROFILE_REFERENCE))
+        pieces = ctx.pieces()
+        ctx.update_voltages(pieces)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        reps = 20
+        for _ in range(reps):
+            ctx.update_voltages(pieces)
+        ms, n = ctx.kernel_time(C.UPDATE_VOLTAGES)
+        avg = ms / n
+        gbs = 24.0 * N / (avg * 1e-3) / 1e9
+        rows.append({"nodes": N, "avg_launch_ms": avg, "achieved_GBs": gbs,
+                     "frac_of_8TBs": gbs / PEAK_HBM_GBS, "bytes_per_launch": 24 * N})
+        log(f"[upv] {rows[-1]}")
+        ctx.close()
+    print(json.dumps({"metric": "update_voltages HBM bandwidth", "unit": "GB/s",
+                      "algorithmic_bytes_per_node": 24, "rows": rows}))
+
+
+def run_copy_calibration(C, nbytes):
+    import numpy as np
+    src = C.DeviceArray(np.zeros(nbytes // 4, np.float32))
+    dst = C.DeviceArray(np.zeros(nbytes // 4, np.float32))
+    for _ in range(5):
+        C.copy_kernel(dst, src, nbytes)
+    C.lib().ckt_device_synchronize(0)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 5}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -100,6 +250,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from legionorigins_amd import circuit as C
+    if args.copy_calibration:
+        return run_copy_calibration(C, args.copy_calibration)
+    if args.batched:
+        return run_batched(C, args)
+    if args.served:
+        return run_served(C, args)
+    if args.upv_sweep:
+        return run_upv_sweep(C, args)
 
     policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
     mode = {"atomic": C.REDUCE_ATOMIC, "csr": C.REDUCE_CSR,

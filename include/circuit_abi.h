@@ -307,6 +307,11 @@ int ckt_batch_destroy(ckt_batch_t* b);
  * footprint does not fit in the remaining budget. */
 int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int32_t num_loops,
                             uint32_t* id);
+/* Generate `n` circuits on `threads` host threads and add them in order;
+ * ids[i] receives the id of params[i].  Stops at the first failure. */
+int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,
+                                 const int32_t* num_loops, uint32_t n, int32_t threads,
+                                 uint32_t* ids);
 int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,
                   const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
                   const ckt_piece_t* pieces, uint32_t num_pieces, int32_t num_loops,

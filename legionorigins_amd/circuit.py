@@ -174,6 +174,8 @@ def _declare(L):
         "ckt_batch_set_timing": ([vp, i32], i32),
         "ckt_batch_kernel_time": ([vp, i32, P(ctypes.c_double), P(u64)], i32),
         "ckt_batch_add_generated": ([vp, P(_GenParams), i32, P(u32)], i32),
+        "ckt_batch_add_generated_many": ([vp, P(_GenParams), P(ctypes.c_int32), u32, i32,
+                                          P(u32)], i32),
         "ckt_batch_add": ([vp, vp, u32, vp, u32, vp, vp, u32, i32, P(u32)], i32),
         "ckt_batch_remove": ([vp, u32], i32),
         "ckt_batch_count": ([vp, P(u32), P(u64)], i32),
@@ -469,6 +471,14 @@ class Batch:
         cp = params.c()
         _check(lib().ckt_batch_add_generated(self.h, ctypes.byref(cp), loops, ctypes.byref(cid)))
         return cid.value
+
+    def add_generated_many(self, params, loops, threads=8):
+        n = len(params)
+        arr = (_GenParams * n)(*[p.c() for p in params])
+        lp = (ctypes.c_int32 * n)(*loops)
+        ids = (ctypes.c_uint32 * n)()
+        _check(lib().ckt_batch_add_generated_many(self.h, arr, lp, n, threads, ids))
+        return list(ids)
 
     def add(self, c: HostCircuit, loops):
         cid = ctypes.c_uint32()

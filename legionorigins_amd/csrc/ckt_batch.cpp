@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ckt_engine.h"
@@ -105,6 +106,45 @@ int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int3
   SlabImage img;
   if ((st = image_from_gen(img, params))) return st;
   return b->eng.add(img, num_loops, id);
+}
+
+int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,
+                                 const int32_t* num_loops, uint32_t n, int32_t threads,
+                                 uint32_t* ids) {
+  int st = check(b);
+  if (st) return st;
+  if (n && (!params || !num_loops || !ids)) return fail(CKT_ERR_INVALID_ARG, "null array");
+  uint64_t need = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t N, W, P;
+    if ((st = gen_validate(&params[i], &N, &W, &P))) return st;
+    if (num_loops[i] < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
+    need += SlabLayout::make(N, W, P).total;
+  }
+  if (b->eng.budget() && b->eng.bytes_in_use() + need > b->eng.budget())
+    return fail(CKT_ERR_BUDGET, "circuits do not fit in the remaining HBM budget");
+  if (threads < 1) threads = 1;
+  // Images are built in windows of `threads` circuits so that host memory
+  // holds at most one window at a time.
+  for (uint32_t base = 0; base < n; base += (uint32_t)threads) {
+    const uint32_t cnt = std::min<uint32_t>((uint32_t)threads, n - base);
+    std::vector<SlabImage> imgs(cnt);
+    std::vector<int> rc(cnt, CKT_OK);
+    std::vector<std::string> err(cnt);
+    std::vector<std::thread> th;
+    for (uint32_t k = 0; k < cnt; k++)
+      th.emplace_back([&, k] {
+        rc[k] = image_from_gen(imgs[k], &params[base + k]);
+        if (rc[k]) err[k] = ckt_last_error();
+      });
+    for (auto& t : th) t.join();
+    for (uint32_t k = 0; k < cnt; k++) {
+      if (rc[k]) return fail(rc[k], err[k]);
+      if ((st = b->eng.add(imgs[k], num_loops[base + k], &ids[base + k]))) return st;
+      imgs[k] = SlabImage();
+    }
+  }
+  return CKT_OK;
 }
 
 int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,

@@ -209,7 +209,10 @@ int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config
       if ((rc = E.nop())) return rc;
       if (cfg.sync == 1 && (rc = E.sync())) return rc;
     }
-    if (cfg.per_piece) {
+    // The CSR reductions sum per node over the whole index space, so in
+    // per-piece mode that phase is still one launch.
+    const bool whole = task == CKT_DISTRIBUTE_CHARGE && cfg.reduce_mode != CKT_REDUCE_ATOMIC;
+    if (cfg.per_piece && !whole) {
       for (const PieceRef& r : all) {
         std::vector<PieceRef> one(1, r);
         if (task == CKT_CALC_NEW_CURRENTS) rc = E.cnc(one, cfg.policy);
