@@ -9,6 +9,7 @@
 #define _GNU_SOURCE
 #include "circuit_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -297,6 +298,64 @@ int orc_calc_new_currents_f64(int seeding, int steps, orc_wire64_t* wires,
   return 0;
 }
 
+int orc_calc_new_currents_f64_limit(orc_wire64_t* wires, const orc_node64_t* nodes,
+                                    uint32_t first_wire, uint32_t num_wires) {
+  const double dt = (double)(float)ORC_DELTAT;
+  enum { NU = 2 * SEG - 1 }; /* I_0..I_9 then v_1..v_9 */
+  int singular = 0;
+  for (uint32_t k = 0; k < num_wires; k++) {
+    orc_wire64_t* w = &wires[first_wire + k];
+    const double vin = nodes[w->in_ptr].voltage, vout = nodes[w->out_ptr].voltage;
+    const double L = w->inductance, R = w->resistance, C = w->capacitance;
+    double A[NU][NU + 1];
+    memset(A, 0, sizeof(A));
+    for (int i = 0; i < SEG; i++) { /* KVL: (L/dt + R) I_i - v_{i+1} + v_i = (L/dt) I0_i */
+      A[i][i] = L / dt + R;
+      A[i][NU] = L / dt * w->current[i];
+      if (i + 1 <= SEG - 1) A[i][SEG + i] -= 1.0;
+      else A[i][NU] += vout;
+      if (i >= 1) A[i][SEG + i - 1] += 1.0;
+      else A[i][NU] -= vin;
+    }
+    for (int m = 1; m < SEG; m++) { /* KCL: (C/dt) v_m - I_{m-1} + I_m = (C/dt) V0_m */
+      const int r = SEG + m - 1;
+      A[r][r] = C / dt;
+      A[r][m - 1] = -1.0;
+      A[r][m] = 1.0;
+      A[r][NU] = C / dt * w->voltage[m - 1];
+    }
+    for (int c = 0; c < NU; c++) { /* elimination with partial pivoting */
+      int piv = c;
+      for (int r = c + 1; r < NU; r++)
+        if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+      if (A[piv][c] == 0.0) {
+        singular++;
+        break;
+      }
+      if (piv != c)
+        for (int j = 0; j <= NU; j++) {
+          double t = A[c][j];
+          A[c][j] = A[piv][j];
+          A[piv][j] = t;
+        }
+      for (int r = c + 1; r < NU; r++) {
+        const double f = A[r][c] / A[c][c];
+        if (f != 0.0)
+          for (int j = c; j <= NU; j++) A[r][j] -= f * A[c][j];
+      }
+    }
+    double x[NU];
+    for (int r = NU - 1; r >= 0; r--) {
+      double acc = A[r][NU];
+      for (int j = r + 1; j < NU; j++) acc -= A[r][j] * x[j];
+      x[r] = acc / A[r][r];
+    }
+    for (int i = 0; i < SEG; i++) w->current[i] = x[i];
+    for (int m = 0; m < SEG - 1; m++) w->voltage[m] = x[SEG + m];
+  }
+  return singular;
+}
+
 /* ------------------------------------------------------------------------ */
 /* distribute_charge                                                          */
 
@@ -457,6 +516,8 @@ static void* cnc_worker(void* arg) {
       if (j->prec == 32)
         orc_calc_new_currents_f32(j->seeding, j->steps, j->wires32, j->nodes32, j->nodes32,
                                   j->nodes32, p->first_wire, p->num_wires);
+      else if (j->prec == 65)
+        orc_calc_new_currents_f64_limit(j->wires64, j->nodes64, p->first_wire, p->num_wires);
       else
         orc_calc_new_currents_f64(j->seeding, j->steps, j->wires64, j->nodes64, p->first_wire,
                                   p->num_wires);
@@ -473,6 +534,8 @@ static void* cnc_worker(void* arg) {
           if (j->prec == 32)
             orc_calc_new_currents_f32(j->seeding, j->steps, j->wires32, j->nodes32, j->nodes32,
                                       j->nodes32, fw, (uint32_t)cnt);
+          else if (j->prec == 65)
+            orc_calc_new_currents_f64_limit(j->wires64, j->nodes64, fw, (uint32_t)cnt);
           else
             orc_calc_new_currents_f64(j->seeding, j->steps, j->wires64, j->nodes64, fw,
                                       (uint32_t)cnt);
@@ -539,6 +602,29 @@ int orc_run_f64(int seeding, int steps, int loops, int nthreads, orc_node64_t* n
     j.seeding = seeding;
     j.steps = steps;
     j.prec = 64;
+    j.wires64 = wires;
+    j.nodes64 = nodes;
+    j.pieces = pieces;
+    j.num_pieces = num_pieces;
+    j.grain = 256;
+    run_cnc_threads(&j, nthreads);
+    for (uint32_t p = 0; p < num_pieces; p++)
+      orc_distribute_charge_f64(wires, nodes, pieces[p].first_wire, pieces[p].num_wires);
+    for (uint32_t p = 0; p < num_pieces; p++)
+      orc_update_voltages_f64(nodes, pieces[p].first_node, pieces[p].num_nodes);
+  }
+  return 0;
+}
+
+int orc_run_f64_limit(int loops, int nthreads, orc_node64_t* nodes, uint32_t num_nodes,
+                      orc_wire64_t* wires, uint32_t num_wires, const orc_piece_t* pieces,
+                      uint32_t num_pieces) {
+  (void)num_nodes;
+  (void)num_wires;
+  for (int loop = 0; loop < loops; loop++) {
+    cnc_job j;
+    memset(&j, 0, sizeof(j));
+    j.prec = 65;
     j.wires64 = wires;
     j.nodes64 = nodes;
     j.pieces = pieces;

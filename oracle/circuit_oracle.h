@@ -145,6 +145,22 @@ int orc_run_f64(int seeding, int steps, int loops, int nthreads, orc_node64_t* n
                 uint32_t num_nodes, orc_wire64_t* wires, uint32_t num_wires,
                 const orc_piece_t* pieces, uint32_t num_pieces);
 
+/* FP64 limit of the relaxation.  The 10000-step iteration of
+ * calculate_currents_task is a fixed-point iteration whose fixed point is the
+ * backward-Euler step of the RLC ladder:
+ *   KVL per segment i:      L (I_i - I0_i)/dt + R I_i = v_{i+1} - v_i
+ *   KCL per internal node k: C (v_k - V0_k)/dt = I_{k-1} - I_k
+ * (v_0, v_10 the in/out node voltages).  In the stable profile it contracts
+ * by < 0.12 per step, so 10000 steps reach the fixed point to the last bit of
+ * FP64.  This solves the 19x19 system per wire (Gaussian elimination with
+ * partial pivoting) instead of iterating; only meaningful when the
+ * relaxation converges.  The loop driver is orc_run_f64's. */
+int orc_calc_new_currents_f64_limit(orc_wire64_t* wires, const orc_node64_t* nodes,
+                                    uint32_t first_wire, uint32_t num_wires);
+int orc_run_f64_limit(int loops, int nthreads, orc_node64_t* nodes, uint32_t num_nodes,
+                      orc_wire64_t* wires, uint32_t num_wires, const orc_piece_t* pieces,
+                      uint32_t num_pieces);
+
 /* calc_new_currents over an explicit list of wire indices on `nthreads`
  * threads (CPU-baseline sample).  Returns wall seconds. */
 double orc_cnc_sample_f32(int seeding, int steps, int nthreads, orc_wire_t* wires,

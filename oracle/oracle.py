@@ -72,6 +72,8 @@ def _declare(L):
     L.orc_run_f32.argtypes = [i32, i32, i32, i32, i32, i32, vp, u32, vp, u32, vp, vp, u32]
     L.orc_run_f32.restype = i32
     L.orc_run_f64.argtypes = [i32, i32, i32, i32, vp, u32, vp, u32, vp, u32]
+    L.orc_calc_new_currents_f64_limit.argtypes = [vp, vp, u32, u32]
+    L.orc_run_f64_limit.argtypes = [i32, i32, vp, u32, vp, u32, vp, u32]
     L.orc_cnc_sample_f32.argtypes = [i32, i32, i32, vp, vp, vp, u32]
     L.orc_cnc_sample_f32.restype = ctypes.c_double
     L.orc_promote.argtypes = [vp, u32, vp, u32, vp, vp]
@@ -176,6 +178,16 @@ def run_f64(c: Circuit, loops, steps=STEPS, seeding=SEED_GPU, nthreads=None):
     n64, w64 = promote(c)
     lib().orc_run_f64(seeding, steps, loops, nthreads, _p(n64), c.num_nodes, _p(w64), c.num_wires,
                       _p(c.pieces), len(c.pieces))
+    return n64, w64
+
+
+def run_f64_limit(c: Circuit, loops, nthreads=None):
+    """FP64 loops with each relaxation replaced by its limit (the backward-
+    Euler solve); valid where the relaxation converges (stable profile)."""
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    n64, w64 = promote(c)
+    lib().orc_run_f64_limit(loops, nthreads, _p(n64), c.num_nodes, _p(w64), c.num_wires,
+                            _p(c.pieces), len(c.pieces))
     return n64, w64
 
 

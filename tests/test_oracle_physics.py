@@ -116,6 +116,17 @@ def test_fixed_point_is_backward_euler_rlc_step(O):
     assert worst32 < 1e-5, worst32
 
 
+def test_fp64_limit_equals_stepwise_fp64(O):
+    # The backward-Euler solve is the limit of the 10000-step relaxation in
+    # the stable profile: the step-by-step FP64 oracle reaches it.
+    c = O.generate(4, 256, 1024, 90, 3, O.PROFILE_STABLE)
+    n1, w1 = O.run_f64(c, 3)
+    n2, w2 = O.run_f64_limit(c, 3)
+    assert np.abs(n1["voltage"] - n2["voltage"]).max() <= 1e-13 * np.abs(n1["voltage"]).max()
+    assert np.abs(w1["current"] - w2["current"]).max() <= 1e-12 * np.abs(w1["current"]).max()
+    assert np.abs(w1["voltage"] - w2["voltage"]).max() <= 1e-12 * np.abs(w1["voltage"]).max()
+
+
 def test_reference_parameters_diverge_with_gain_L_over_dt_R(O):
     c = O.generate(1, 256, 200, 100, 12345, O.PROFILE_REFERENCE)
     n64, w64a = O.promote(c)
