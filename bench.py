@@ -331,12 +331,25 @@ def main():
             log(f"[bench] policy {name}: {pol[name]}")
         detail["policies"] = pol
         red = {}
-        for name, mv in (("atomic", C.REDUCE_ATOMIC), ("csr", C.REDUCE_CSR),
-                         ("kahan64", C.REDUCE_CSR_KAHAN64)):
+        for name, mv, impl in (("atomic_auto", C.REDUCE_ATOMIC, 0),
+                               ("atomic_global", C.REDUCE_ATOMIC, 1),
+                               ("atomic_lds", C.REDUCE_ATOMIC, 2), ("csr", C.REDUCE_CSR, 0),
+                               ("kahan64", C.REDUCE_CSR_KAHAN64, 0)):
             ctx.set_option(C.OPT_STEPS, 1)
+            ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, impl)
+            ctx.set_option(C.OPT_FUSE_DSC, 0)
             r3, w3, k3 = time_loops(C, ctx, 5, C.POLICY_FAST, mv)
             ctx.set_option(C.OPT_STEPS, C.STEPS)
+            ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, 0)
+            ctx.set_option(C.OPT_FUSE_DSC, 1)
             red[name] = {"distribute_charge_ms": k3[C.DISTRIBUTE_CHARGE][0]}
+        # the full loop with distribute_charge as its own launch
+        ctx.set_option(C.OPT_FUSE_DSC, 0)
+        r4, w4, k4 = time_loops(C, ctx, 4, policy, mode)
+        ctx.set_option(C.OPT_FUSE_DSC, 1)
+        red["unfused_loop"] = {"wire_updates_per_s": W * 4 / w4,
+                               "calc_new_currents_ms": k4[C.CALC_NEW_CURRENTS][0],
+                               "distribute_charge_ms": k4[C.DISTRIBUTE_CHARGE][0]}
         detail["distribute_charge"] = red
         log(f"[bench] reductions: {red}")
     out["detail"] = detail

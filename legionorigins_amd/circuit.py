@@ -32,7 +32,8 @@ REDUCE_ID = 1
 POLICY_EXACT, POLICY_RECIP, POLICY_FAST = 0, 1, 2
 REDUCE_ATOMIC, REDUCE_CSR, REDUCE_CSR_KAHAN64 = 0, 1, 2
 PROFILE_REFERENCE, PROFILE_STABLE = 0, 1
-OPT_STEPS, OPT_CNC_BLOCK, OPT_PERSISTENT, OPT_WAVES_PER_SIMD, OPT_DSC_ATOMIC_IMPL = range(5)
+(OPT_STEPS, OPT_CNC_BLOCK, OPT_PERSISTENT, OPT_WAVES_PER_SIMD, OPT_DSC_ATOMIC_IMPL,
+ OPT_FUSE_DSC) = range(6)
 
 # CircuitNode / CircuitWire / CircuitPiece records (circuit.h:35-70).
 NODE = np.dtype([("charge", "<f4"), ("voltage", "<f4"), ("capacitance", "<f4"),

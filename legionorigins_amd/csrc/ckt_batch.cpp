@@ -195,9 +195,7 @@ int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* ac
     if (cfg.use_graph) {
       if ((st = E.run_graph(refs, cfg.policy, cfg.reduce_mode, 1))) return st;
     } else {
-      if ((st = E.cnc(refs, cfg.policy))) return st;
-      if ((st = E.dsc(refs, cfg.reduce_mode))) return st;
-      if ((st = E.upv(refs))) return st;
+      if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) return st;
     }
   }
   int32_t n = 0;
@@ -234,11 +232,8 @@ int ckt_batch_run(ckt_batch_t* b, const ckt_batch_config_t* config, ckt_batch_re
     if (cfg.use_graph) {
       if ((st = E.run_graph(refs, cfg.policy, cfg.reduce_mode, k))) return st;
     } else {
-      for (int32_t i = 0; i < k; i++) {
-        if ((st = E.cnc(refs, cfg.policy))) return st;
-        if ((st = E.dsc(refs, cfg.reduce_mode))) return st;
-        if ((st = E.upv(refs))) return st;
-      }
+      for (int32_t i = 0; i < k; i++)
+        if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) return st;
     }
     for (Circuit* c : E.slots())
       if (c->loops_left > 0) {

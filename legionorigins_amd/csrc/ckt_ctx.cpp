@@ -235,6 +235,9 @@ int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config
   const double t0 = now_s();
   if (cfg.use_graph && !cfg.sync && !cfg.per_piece) {
     if ((st = E.run_graph(all, cfg.policy, cfg.reduce_mode, num_loops))) return st;
+  } else if (!cfg.sync && !cfg.per_piece) {
+    for (int i = 0; i < num_loops; i++)
+      if ((st = E.loop_once(all, cfg.policy, cfg.reduce_mode))) return st;
   } else {
     for (int i = 0; i < num_loops; i++) {
       if ((st = phase(CKT_CALC_NEW_CURRENTS_NOP, CKT_CALC_NEW_CURRENTS))) return st;

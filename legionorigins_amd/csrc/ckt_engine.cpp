@@ -590,7 +590,7 @@ void Engine::reset_timing() {
   }
 }
 
-int Engine::cnc(const std::vector<PieceRef>& refs, int policy) {
+int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
   if (policy < CKT_POLICY_EXACT || policy > CKT_POLICY_FAST)
     return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: unknown policy");
   int st;
@@ -604,13 +604,24 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy) {
     const int waves_per_block = cnc_block_ / 64;
     int grid = num_cus_ * 4 * waves_per_simd_ / waves_per_block;
     if (grid > (int)t.n) grid = (int)t.n;
-    e = launch_cnc_persistent(policy, d_table_, (const Tile*)t.p, t.n, cnc_block_, grid, d_counters_,
-                              steps_, stream_);
+    e = launch_cnc_persistent(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, cnc_block_, grid,
+                              d_counters_, steps_, stream_);
   } else {
-    e = launch_cnc(policy, d_table_, (const Tile*)t.p, t.n, cnc_block_, steps_, stream_);
+    e = launch_cnc(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, cnc_block_, steps_, stream_);
   }
   if (e != hipSuccess) return hip_fail(e, "calc_new_currents launch");
   return timed_end(CKT_CALC_NEW_CURRENTS, a);
+}
+
+int Engine::loop_once(const std::vector<PieceRef>& refs, int policy, int mode) {
+  int st;
+  if (fuse_dsc_ && mode == CKT_REDUCE_ATOMIC) {
+    if ((st = cnc(refs, policy, true))) return st;
+  } else {
+    if ((st = cnc(refs, policy))) return st;
+    if ((st = dsc(refs, mode))) return st;
+  }
+  return upv(refs);
 }
 
 int Engine::csr_nodes_ok(const std::vector<PieceRef>& refs) const {
@@ -710,8 +721,8 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
   int st;
   if ((st = ensure_table())) return st;
   uint64_t key = fnv(1469598103934665603ull, refs.data(), refs.size() * sizeof(PieceRef));
-  const int64_t cfg[8] = {policy, mode, steps_, cnc_block_, persistent_, waves_per_simd_,
-                          dsc_atomic_impl_, (int64_t)generation_};
+  const int64_t cfg[9] = {policy, mode, steps_, cnc_block_, persistent_, waves_per_simd_,
+                          dsc_atomic_impl_, fuse_dsc_, (int64_t)generation_};
   key = fnv(key, cfg, sizeof(cfg));
   auto it = graphs_.find(key);
   hipGraphExec_t exec = nullptr;
@@ -729,9 +740,7 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
     const bool timing = timing_;
     timing_ = false;
     CKT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    st = cnc(refs, policy);
-    if (!st) st = dsc(refs, mode);
-    if (!st) st = upv(refs);
+    st = loop_once(refs, policy, mode);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(stream_, &graph);
     timing_ = timing;
@@ -771,6 +780,7 @@ int Engine::set_option(int opt, int64_t v) {
       if (v < 0 || v > 2) return fail(CKT_ERR_INVALID_ARG, "atomic implementation must be 0, 1 or 2");
       dsc_atomic_impl_ = (int)v;
       return CKT_OK;
+    case OPT_FUSE_DSC: fuse_dsc_ = v ? 1 : 0; return CKT_OK;
     default: return fail(CKT_ERR_INVALID_ARG, "unknown option");
   }
 }
@@ -782,6 +792,7 @@ int64_t Engine::option(int opt) const {
     case OPT_PERSISTENT: return persistent_;
     case OPT_WAVES_PER_SIMD: return waves_per_simd_;
     case OPT_DSC_ATOMIC_IMPL: return dsc_atomic_impl_;
+    case OPT_FUSE_DSC: return fuse_dsc_;
     default: return -1;
   }
 }

@@ -69,6 +69,7 @@ enum EngineOption {
   OPT_PERSISTENT = 2,
   OPT_WAVES_PER_SIMD = 3,
   OPT_DSC_ATOMIC_IMPL = 4,  // 0 auto, 1 global atomics, 2 LDS per piece
+  OPT_FUSE_DSC = 5,         // loop drivers: ATOMIC distribute_charge fused into calc_new_currents
 };
 
 class Engine {
@@ -99,7 +100,7 @@ class Engine {
   void all_refs(std::vector<PieceRef>& refs, bool only_active) const;
 
   // Task launches (asynchronous on the engine stream).
-  int cnc(const std::vector<PieceRef>& refs, int policy);
+  int cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc = false);
   int dsc(const std::vector<PieceRef>& refs, int mode);
   int upv(const std::vector<PieceRef>& refs);
   int sanity(const std::vector<PieceRef>& refs, uint64_t* bad);
@@ -109,6 +110,10 @@ class Engine {
   // One loop (cnc, dsc, upv) of `refs` captured in a graph and replayed
   // `times` times.
   int run_graph(const std::vector<PieceRef>& refs, int policy, int mode, int times);
+  // One loop of `refs`: calc_new_currents, distribute_charge, update_voltages
+  // (the first two as one fused launch when OPT_FUSE_DSC is on and the mode
+  // is ATOMIC).
+  int loop_once(const std::vector<PieceRef>& refs, int policy, int mode);
 
   int set_option(int opt, int64_t value);
   int64_t option(int opt) const;
@@ -156,6 +161,7 @@ class Engine {
   int persistent_ = 0;
   int waves_per_simd_ = 8;
   int dsc_atomic_impl_ = 0;
+  int fuse_dsc_ = 1;
   bool timing_ = false;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_;

@@ -54,11 +54,13 @@ struct PieceTile {
 };
 
 // Host launchers (defined in ckt_kernels.hip).  All are asynchronous on `s`.
-hipError_t launch_cnc(int policy, const DevCircuit* circuits, const Tile* tiles, uint32_t ntiles,
-                      int block, int steps, hipStream_t s);
-hipError_t launch_cnc_persistent(int policy, const DevCircuit* circuits, const Tile* tiles,
-                                 uint32_t ntiles, int block, int grid, uint32_t* counters,
-                                 int steps, hipStream_t s);
+// fuse_dsc adds each wire's two charge contributions with float atomics at the
+// end of the relaxation (the ATOMIC distribute_charge, fused).
+hipError_t launch_cnc(int policy, bool fuse_dsc, const DevCircuit* circuits, const Tile* tiles,
+                      uint32_t ntiles, int block, int steps, hipStream_t s);
+hipError_t launch_cnc_persistent(int policy, bool fuse_dsc, const DevCircuit* circuits,
+                                 const Tile* tiles, uint32_t ntiles, int block, int grid,
+                                 uint32_t* counters, int steps, hipStream_t s);
 hipError_t launch_dsc_atomic(const DevCircuit* circuits, const Tile* tiles, uint32_t ntiles,
                              int block, hipStream_t s);
 hipError_t launch_dsc_lds(const DevCircuit* circuits, const PieceTile* ptiles, uint32_t n,

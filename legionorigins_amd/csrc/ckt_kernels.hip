@@ -145,7 +145,10 @@ struct Relax<CKT_POLICY_FAST> {
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
 
-template <int POLICY>
+// FUSE: the wire's two charge contributions (distribute_charge's ATOMIC
+// mode, circuit_gpu.cu:273-283) are added from registers right after the
+// relaxation, so the loop drivers skip the separate launch.
+template <int POLICY, bool FUSE>
 __device__ __forceinline__ void cnc_one(const DevCircuit& c, uint32_t w, int steps) {
   const size_t ws = c.ws;
   gfloat* cur = (gfloat*)c.cur;
@@ -164,20 +167,25 @@ __device__ __forceinline__ void cnc_one(const DevCircuit& c, uint32_t w, int ste
   for (int s = 0; s < kSeg; s++) cur[s * ws + w] = I[s];
 #pragma unroll
   for (int s = 0; s < kSeg - 1; s++) vol[s * ws + w] = V[s];
+  if (FUSE) {
+    const float dt = CKT_DT_F;
+    unsafeAtomicAdd(&c.charge[c.in_ptr[w]], -dt * I[0]);
+    unsafeAtomicAdd(&c.charge[c.out_ptr[w]], dt * I[kSeg - 1]);
+  }
 }
 
-template <int POLICY, int BLOCK>
+template <int POLICY, int BLOCK, bool FUSE>
 __global__ __launch_bounds__(BLOCK) void cnc_kernel(const DevCircuit* __restrict__ circuits,
                                                     const Tile* __restrict__ tiles, int steps) {
   const Tile t = tiles[blockIdx.x];
   if (threadIdx.x >= t.count) return;
-  cnc_one<POLICY>(circuits[t.circuit], t.first + threadIdx.x, steps);
+  cnc_one<POLICY, FUSE>(circuits[t.circuit], t.first + threadIdx.x, steps);
 }
 
 // Persistent form: a fixed grid pulls tiles from a queue counter.  The last
 // block to finish resets the two counters, so consecutive launches on one
 // stream reuse them without a separate clear.
-template <int POLICY, int BLOCK>
+template <int POLICY, int BLOCK, bool FUSE>
 __global__ __launch_bounds__(BLOCK) void cnc_persistent_kernel(const DevCircuit* __restrict__ circuits,
                                                                const Tile* __restrict__ tiles,
                                                                uint32_t ntiles, uint32_t* counters,
@@ -190,7 +198,8 @@ __global__ __launch_bounds__(BLOCK) void cnc_persistent_kernel(const DevCircuit*
     __syncthreads();
     if (ti >= ntiles) break;
     const Tile t = tiles[ti];
-    if (threadIdx.x < t.count) cnc_one<POLICY>(circuits[t.circuit], t.first + threadIdx.x, steps);
+    if (threadIdx.x < t.count)
+      cnc_one<POLICY, FUSE>(circuits[t.circuit], t.first + threadIdx.x, steps);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -466,53 +475,64 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst,
 
 static inline uint32_t blocks_for(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
 
-template <int POLICY>
-static hipError_t cnc_dispatch(const DevCircuit* c, const Tile* t, uint32_t n, int block, int steps,
-                               hipStream_t s) {
+// Plain grid (one tile per workgroup) or persistent grid (`grid` workgroups
+// pulling tiles), for every policy x workgroup size x fused epilogue.
+template <int POLICY, int BLOCK, bool FUSE>
+static void cnc_go(bool persistent, const DevCircuit* c, const Tile* t, uint32_t n, int grid,
+                   uint32_t* ctr, int steps, hipStream_t s) {
+  if (persistent)
+    hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, BLOCK, FUSE>), dim3(grid), dim3(BLOCK), 0, s,
+                       c, t, n, ctr, steps);
+  else
+    hipLaunchKernelGGL((cnc_kernel<POLICY, BLOCK, FUSE>), dim3(n), dim3(BLOCK), 0, s, c, t, steps);
+}
+
+template <int POLICY, bool FUSE>
+static hipError_t cnc_by_block(bool persistent, const DevCircuit* c, const Tile* t, uint32_t n,
+                               int block, int grid, uint32_t* ctr, int steps, hipStream_t s) {
   switch (block) {
-    case 64: hipLaunchKernelGGL((cnc_kernel<POLICY, 64>), dim3(n), dim3(64), 0, s, c, t, steps); break;
-    case 128: hipLaunchKernelGGL((cnc_kernel<POLICY, 128>), dim3(n), dim3(128), 0, s, c, t, steps); break;
-    case 256: hipLaunchKernelGGL((cnc_kernel<POLICY, 256>), dim3(n), dim3(256), 0, s, c, t, steps); break;
-    case 512: hipLaunchKernelGGL((cnc_kernel<POLICY, 512>), dim3(n), dim3(512), 0, s, c, t, steps); break;
+    case 64: cnc_go<POLICY, 64, FUSE>(persistent, c, t, n, grid, ctr, steps, s); break;
+    case 128: cnc_go<POLICY, 128, FUSE>(persistent, c, t, n, grid, ctr, steps, s); break;
+    case 256: cnc_go<POLICY, 256, FUSE>(persistent, c, t, n, grid, ctr, steps, s); break;
+    case 512: cnc_go<POLICY, 512, FUSE>(persistent, c, t, n, grid, ctr, steps, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_cnc(int policy, const DevCircuit* c, const Tile* t, uint32_t n, int block,
-                      int steps, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  switch (policy) {
-    case CKT_POLICY_EXACT: return cnc_dispatch<CKT_POLICY_EXACT>(c, t, n, block, steps, s);
-    case CKT_POLICY_RECIP: return cnc_dispatch<CKT_POLICY_RECIP>(c, t, n, block, steps, s);
-    case CKT_POLICY_FAST: return cnc_dispatch<CKT_POLICY_FAST>(c, t, n, block, steps, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
 template <int POLICY>
-static hipError_t cncp_dispatch(const DevCircuit* c, const Tile* t, uint32_t n, int block, int grid,
-                                uint32_t* ctr, int steps, hipStream_t s) {
-  switch (block) {
-    case 64: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 64>), dim3(grid), dim3(64), 0, s, c, t, n, ctr, steps); break;
-    case 128: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 128>), dim3(grid), dim3(128), 0, s, c, t, n, ctr, steps); break;
-    case 256: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 256>), dim3(grid), dim3(256), 0, s, c, t, n, ctr, steps); break;
-    case 512: hipLaunchKernelGGL((cnc_persistent_kernel<POLICY, 512>), dim3(grid), dim3(512), 0, s, c, t, n, ctr, steps); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+static hipError_t cnc_by_fuse(bool fuse, bool persistent, const DevCircuit* c, const Tile* t,
+                              uint32_t n, int block, int grid, uint32_t* ctr, int steps,
+                              hipStream_t s) {
+  return fuse ? cnc_by_block<POLICY, true>(persistent, c, t, n, block, grid, ctr, steps, s)
+              : cnc_by_block<POLICY, false>(persistent, c, t, n, block, grid, ctr, steps, s);
 }
 
-hipError_t launch_cnc_persistent(int policy, const DevCircuit* c, const Tile* t, uint32_t n,
-                                 int block, int grid, uint32_t* ctr, int steps, hipStream_t s) {
+static hipError_t cnc_launch(int policy, bool fuse, bool persistent, const DevCircuit* c,
+                             const Tile* t, uint32_t n, int block, int grid, uint32_t* ctr,
+                             int steps, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (grid < 1) return hipErrorInvalidValue;
+  if (persistent && grid < 1) return hipErrorInvalidValue;
   switch (policy) {
-    case CKT_POLICY_EXACT: return cncp_dispatch<CKT_POLICY_EXACT>(c, t, n, block, grid, ctr, steps, s);
-    case CKT_POLICY_RECIP: return cncp_dispatch<CKT_POLICY_RECIP>(c, t, n, block, grid, ctr, steps, s);
-    case CKT_POLICY_FAST: return cncp_dispatch<CKT_POLICY_FAST>(c, t, n, block, grid, ctr, steps, s);
+    case CKT_POLICY_EXACT:
+      return cnc_by_fuse<CKT_POLICY_EXACT>(fuse, persistent, c, t, n, block, grid, ctr, steps, s);
+    case CKT_POLICY_RECIP:
+      return cnc_by_fuse<CKT_POLICY_RECIP>(fuse, persistent, c, t, n, block, grid, ctr, steps, s);
+    case CKT_POLICY_FAST:
+      return cnc_by_fuse<CKT_POLICY_FAST>(fuse, persistent, c, t, n, block, grid, ctr, steps, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_cnc(int policy, bool fuse_dsc, const DevCircuit* c, const Tile* t, uint32_t n,
+                      int block, int steps, hipStream_t s) {
+  return cnc_launch(policy, fuse_dsc, false, c, t, n, block, 0, nullptr, steps, s);
+}
+
+hipError_t launch_cnc_persistent(int policy, bool fuse_dsc, const DevCircuit* c, const Tile* t,
+                                 uint32_t n, int block, int grid, uint32_t* ctr, int steps,
+                                 hipStream_t s) {
+  return cnc_launch(policy, fuse_dsc, true, c, t, n, block, grid, ctr, steps, s);
 }
 
 hipError_t launch_dsc_atomic(const DevCircuit* c, const Tile* t, uint32_t n, int block,

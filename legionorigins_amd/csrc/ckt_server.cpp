@@ -283,9 +283,7 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
       for (uint32_t k = 0; k < c->num_pieces; k++) refs.push_back(PieceRef{slot, k});
     }
     if (!refs.empty()) {
-      if ((st = E.cnc(refs, cfg.policy))) break;
-      if ((st = E.dsc(refs, cfg.reduce_mode))) break;
-      if ((st = E.upv(refs))) break;
+      if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) break;
       iterations++;
     }
     // Overlap: upload circuits whose images are ready while the GPU works.

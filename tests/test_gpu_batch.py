@@ -100,9 +100,10 @@ def test_autotune_restores_state(C):
 
 def test_atomic_implementations_agree(C):
     outs = []
-    for impl in (1, 2):
+    for fuse, impl in ((0, 1), (0, 2), (1, 0)):
         with C.Context(0) as ctx:
             ctx.generate(C.GenParams(8, 2048, 8192, 80, 9, C.PROFILE_STABLE))
+            ctx.set_option(C.OPT_FUSE_DSC, fuse)
             ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, impl)
             ctx.run_loops(2, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC)
             outs.append(ctx.store()[0])

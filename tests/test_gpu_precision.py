@@ -49,3 +49,41 @@ def test_c3_precision_study(C, O, pct):
     for k, r in rows.items():
         assert r["err_vs_fp32_reference"] <= TOL, k
         assert r["fp64_err_all"] <= 1e-3, k
+
+
+def test_reduction_error_in_ulps(C):
+    """The node voltages above are dominated by update_voltages' own
+    rounding; this isolates the reductions: after one calc_new_currents
+    (EXACT) and one distribute_charge, each node's charge is compared with the
+    correctly rounded exact sum of the float contributions -dt*I[0], +dt*I[9]."""
+    import math
+    params = C.GenParams(4, 1024, 4096, 50, 12345, C.PROFILE_STABLE)
+    dt = np.float32(1e-6)
+    out = {}
+    for mode, name in ((C.REDUCE_CSR, "csr"), (C.REDUCE_ATOMIC, "atomic"),
+                       (C.REDUCE_CSR_KAHAN64, "csr_kahan64")):
+        with C.Context(0) as ctx:
+            ctx.generate(params)
+            pieces = ctx.pieces()
+            ctx.calc_new_currents(pieces, C.POLICY_EXACT)
+            ctx.distribute_charge(pieces, mode)
+            nodes, wires = ctx.store()
+        out[name] = nodes["charge"]
+    vals = [[] for _ in range(len(nodes))]
+    q_in = (-dt * wires["current"][:, 0]).astype(np.float32)
+    q_out = (dt * wires["current"][:, 9]).astype(np.float32)
+    for k in range(len(wires)):
+        vals[wires["in_ptr"][k]].append(float(q_in[k]))
+        vals[wires["out_ptr"][k]].append(float(q_out[k]))
+    exact = np.array([math.fsum(v) for v in vals])
+    want = exact.astype(np.float32)
+    ulp = np.spacing(np.maximum(np.abs(want), np.float32(1e-30)))
+    host = C.generate(params)
+    shared = host.locator == 0
+    print(f"\nreduction error after one loop (pct 50, C1), max |charge - exact| in ulps of the exact sum")
+    for name, q in out.items():
+        e = np.abs(q.astype(np.float64) - exact) / ulp
+        print(f"{name:12s} shared {e[shared].max():8.2f}   private {e[~shared].max():8.2f}")
+        out[name] = e
+    assert out["csr_kahan64"][shared].max() <= 0.5 + 1e-9
+    assert np.array_equal(out["csr_kahan64"][~shared], out["csr"][~shared])
