@@ -488,6 +488,8 @@ int Engine::upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n,
 }
 
 enum TileKind { TK_WIRES = 1, TK_NODES = 2, TK_CSR = 3, TK_PIECES = 4 };
+// update_voltages: 256 lanes x 4 nodes per tile.
+constexpr int kUpvTile = 1024;
 
 int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out) {
   uint64_t key = fnv(1469598103934665603ull, &kind, sizeof(kind));
@@ -682,7 +684,7 @@ int Engine::upv(const std::vector<PieceRef>& refs) {
   int st;
   if ((st = ensure_table())) return st;
   DevBuf t;
-  if ((st = tiles_for(TK_NODES, refs, 256, &t))) return st;
+  if ((st = tiles_for(TK_NODES, refs, kUpvTile, &t))) return st;
   hipEvent_t a;
   if ((st = timed_begin(CKT_UPDATE_VOLTAGES, &a))) return st;
   hipError_t e = launch_upv(d_table_, (const Tile*)t.p, t.n, 256, stream_);
@@ -732,7 +734,7 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
     if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
     if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
     if ((st = tiles_for(TK_PIECES, refs, 0, &t))) return st;
-    if ((st = tiles_for(TK_NODES, refs, 256, &t))) return st;
+    if ((st = tiles_for(TK_NODES, refs, kUpvTile, &t))) return st;
     if (mode != CKT_REDUCE_ATOMIC) {
       if ((st = csr_nodes_ok(refs))) return st;
       if ((st = tiles_for(TK_CSR, refs, 256, &t))) return st;

@@ -303,18 +303,39 @@ __global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restri
 // ---------------------------------------------------------------------------
 // update_voltages (circuit_gpu.cu:330-345; circuit.cc:1299-1302).
 
+__device__ __forceinline__ float upv_one(float v, float q, float C, float lk) {
+  v = v + q / C;
+  return v * (1.f - lk);
+}
+
+// Tiles of up to 4 * 256 nodes: each lane updates 4 consecutive nodes with
+// 128-bit loads and stores when the tile starts on a 16-byte boundary (node
+// arrays start 256-byte aligned), one node at a time otherwise.
 __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__ circuits,
                                                   const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
-  if (threadIdx.x >= t.count) return;
+  const uint32_t k0 = 4u * threadIdx.x;
+  if (k0 >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
-  const uint32_t n = t.first + threadIdx.x;
-  const float q = c.charge[n], C = c.ncap[n], lk = c.leak[n];
-  float v = c.voltage[n];
-  v = v + q / C;
-  v = v * (1.f - lk);
-  c.voltage[n] = v;
-  c.charge[n] = 0.f;
+  const uint32_t n = t.first + k0;
+  if ((t.first & 3u) == 0 && k0 + 4 <= t.count) {
+    const float4 q = *(const float4*)(c.charge + n);
+    const float4 C = *(const float4*)(c.ncap + n);
+    const float4 lk = *(const float4*)(c.leak + n);
+    float4 v = *(const float4*)(c.voltage + n);
+    v.x = upv_one(v.x, q.x, C.x, lk.x);
+    v.y = upv_one(v.y, q.y, C.y, lk.y);
+    v.z = upv_one(v.z, q.z, C.z, lk.z);
+    v.w = upv_one(v.w, q.w, C.w, lk.w);
+    *(float4*)(c.voltage + n) = v;
+    *(float4*)(c.charge + n) = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    const uint32_t end = t.count - k0 < 4u ? t.count - k0 : 4u;
+    for (uint32_t k = 0; k < end; k++) {
+      c.voltage[n + k] = upv_one(c.voltage[n + k], c.charge[n + k], c.ncap[n + k], c.leak[n + k]);
+      c.charge[n + k] = 0.f;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
