@@ -729,7 +729,10 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
   auto it = graphs_.find(key);
   hipGraphExec_t exec = nullptr;
   if (it == graphs_.end()) {
-    // Every buffer the captured launches use is created before capture.
+    // Every buffer the captured launches use is created before capture, in a
+    // cache with room for all of them (a full cache would evict entries made
+    // here and force an allocation during capture).
+    if (tile_cache_.size() + 8 > 64) drop_caches();
     DevBuf t;
     if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
     if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;

@@ -318,22 +318,28 @@ __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__
   if (k0 >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
   const uint32_t n = t.first + k0;
+  gfloat* charge = (gfloat*)c.charge;
+  gfloat* voltage = (gfloat*)c.voltage;
+  gfloat* ncap = (gfloat*)c.ncap;
+  gfloat* leak = (gfloat*)c.leak;
   if ((t.first & 3u) == 0 && k0 + 4 <= t.count) {
-    const float4 q = *(const float4*)(c.charge + n);
-    const float4 C = *(const float4*)(c.ncap + n);
-    const float4 lk = *(const float4*)(c.leak + n);
-    float4 v = *(const float4*)(c.voltage + n);
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) v4f gv4f;
+    const v4f q = *(const gv4f*)(charge + n);
+    const v4f C = *(const gv4f*)(ncap + n);
+    const v4f lk = *(const gv4f*)(leak + n);
+    v4f v = *(gv4f*)(voltage + n);
     v.x = upv_one(v.x, q.x, C.x, lk.x);
     v.y = upv_one(v.y, q.y, C.y, lk.y);
     v.z = upv_one(v.z, q.z, C.z, lk.z);
     v.w = upv_one(v.w, q.w, C.w, lk.w);
-    *(float4*)(c.voltage + n) = v;
-    *(float4*)(c.charge + n) = make_float4(0.f, 0.f, 0.f, 0.f);
+    *(gv4f*)(voltage + n) = v;
+    *(gv4f*)(charge + n) = (v4f){0.f, 0.f, 0.f, 0.f};
   } else {
     const uint32_t end = t.count - k0 < 4u ? t.count - k0 : 4u;
     for (uint32_t k = 0; k < end; k++) {
-      c.voltage[n + k] = upv_one(c.voltage[n + k], c.charge[n + k], c.ncap[n + k], c.leak[n + k]);
-      c.charge[n + k] = 0.f;
+      voltage[n + k] = upv_one(voltage[n + k], charge[n + k], ncap[n + k], leak[n + k]);
+      charge[n + k] = 0.f;
     }
   }
 }
