@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-cap-wires", type=int, default=8 << 20)
+    ap.add_argument("--serve-timeout", type=float, default=0.0,
+                    help="served: admission deadline per request in seconds (0 = none)")
     ap.add_argument("--compare-seconds", type=float, default=2.0,
                     help="served: trace length of the batched-vs-sequential comparison")
     ap.add_argument("--upv-sweep", action="store_true",
@@ -168,7 +170,8 @@ def run_batched(C, args):
 
 def run_served(C, args):
     budget = 200 * 10**9
-    trace = C.poisson_trace(args.serve_rate, args.serve_seconds, seed=5)
+    trace = C.poisson_trace(args.serve_rate, args.serve_seconds, seed=5,
+                            timeout_s=args.serve_timeout)
     offered = sum(r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops for r in trace)
     log(f"[served] {len(trace)} requests over {args.serve_seconds} s, "
         f"{offered:.3g} wire-updates offered ({offered / args.serve_seconds:.3g}/s)")
@@ -201,6 +204,7 @@ def run_served(C, args):
            "latency_mean_s": rep["latency_mean_s"], "latency_max_s": rep["latency_max_s"],
            "offered_wire_updates_per_s": rep["offered_wire_updates_per_s"],
            "completed": rep["completed"], "rejected": rep["rejected"],
+           "timed_out": rep["timed_out"], "admission_timeout_s": args.serve_timeout,
            "iterations": rep["iterations"], "max_resident": rep["max_resident"],
            "elapsed_s": rep["elapsed_s"],
            "compare_short_trace": {"seconds": args.compare_seconds, **cmp,

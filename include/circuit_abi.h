@@ -354,6 +354,8 @@ typedef struct ckt_request {
   ckt_gen_params_t gen;
   int32_t num_loops;
   double arrival_s;            /* offset from the start of the trace */
+  double timeout_s;            /* > 0: not admitted within this many seconds of
+                                  arrival -> dropped with CKT_ERR_TIMEOUT */
 } ckt_request_t;
 
 typedef struct ckt_request_result {
@@ -371,6 +373,7 @@ typedef struct ckt_server_report {
   double offered_wire_updates_per_s;
   double latency_p50_s, latency_p99_s, latency_mean_s, latency_max_s;
   int32_t completed, rejected, iterations, max_resident;
+  int32_t timed_out;           /* dropped at their admission deadline */
 } ckt_server_report_t;
 
 int ckt_server_create(int device, const ckt_server_config_t* config, ckt_server_t** out);

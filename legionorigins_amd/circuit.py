@@ -93,7 +93,8 @@ class _ServerConfig(ctypes.Structure):
 
 
 class _Request(ctypes.Structure):
-    _fields_ = [("gen", _GenParams), ("num_loops", ctypes.c_int32), ("arrival_s", ctypes.c_double)]
+    _fields_ = [("gen", _GenParams), ("num_loops", ctypes.c_int32), ("arrival_s", ctypes.c_double),
+                ("timeout_s", ctypes.c_double)]
 
 
 class _RequestResult(ctypes.Structure):
@@ -110,7 +111,8 @@ class _ServerReport(ctypes.Structure):
                 ("latency_p50_s", ctypes.c_double), ("latency_p99_s", ctypes.c_double),
                 ("latency_mean_s", ctypes.c_double), ("latency_max_s", ctypes.c_double),
                 ("completed", ctypes.c_int32), ("rejected", ctypes.c_int32),
-                ("iterations", ctypes.c_int32), ("max_resident", ctypes.c_int32)]
+                ("iterations", ctypes.c_int32), ("max_resident", ctypes.c_int32),
+                ("timed_out", ctypes.c_int32)]
 
 
 _lib = None
@@ -551,6 +553,7 @@ class Request:
     gen: GenParams
     num_loops: int
     arrival_s: float
+    timeout_s: float = 0.0
 
 
 class Server:
@@ -581,7 +584,7 @@ class Server:
         n = len(requests)
         reqs = (_Request * n)()
         for i, r in enumerate(requests):
-            reqs[i] = _Request(r.gen.c(), r.num_loops, r.arrival_s)
+            reqs[i] = _Request(r.gen.c(), r.num_loops, r.arrival_s, r.timeout_s)
         res = (_RequestResult * n)()
         rep = _ServerReport()
         _check(lib().ckt_server_run_trace(self.h, reqs, n, res, ctypes.byref(rep)))
@@ -599,10 +602,11 @@ class Server:
 
 
 def poisson_trace(rate_per_s, duration_s, seed, wires_lo=2**12, wires_hi=2**20, loops_lo=1,
-                  loops_hi=20, pct=95, profile=PROFILE_REFERENCE, base_seed=1000):
+                  loops_hi=20, pct=95, profile=PROFILE_REFERENCE, base_seed=1000, timeout_s=0.0):
     """The served workload: Poisson arrivals; per request wires log-uniform in
     [wires_lo, wires_hi], nodes = wires/4, pieces = ceil(nodes/4096), uniform
-    pieces (remainders dropped), loops uniform in [loops_lo, loops_hi]."""
+    pieces (remainders dropped), loops uniform in [loops_lo, loops_hi];
+    optional admission timeout per request."""
     rng = np.random.default_rng(seed)
     t, out, i = 0.0, [], 0
     while True:
@@ -611,7 +615,7 @@ def poisson_trace(rate_per_s, duration_s, seed, wires_lo=2**12, wires_hi=2**20, 
             break
         W = int(np.exp(rng.uniform(np.log(wires_lo), np.log(wires_hi))))
         out.append(Request(shape_for_wires(W, pct, base_seed + i, profile),
-                           int(rng.integers(loops_lo, loops_hi + 1)), t))
+                           int(rng.integers(loops_lo, loops_hi + 1)), t, timeout_s))
         i += 1
     return out
 

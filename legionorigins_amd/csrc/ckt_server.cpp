@@ -225,7 +225,7 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
   std::vector<int64_t> circuit_of(n, -1);  // request -> circuit id
   std::vector<uint32_t> request_of_circuit;
   std::vector<int32_t> loops_left(n, 0);
-  uint32_t finished = 0, rejected = 0;
+  uint32_t finished = 0, rejected = 0, timed_out = 0;
   int32_t iterations = 0, max_resident = 0;
   double wire_updates = 0;
   std::vector<double> latencies;
@@ -258,6 +258,19 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
     while (next_arrival < order.size() && reqs[order[next_arrival]].arrival_s <= now) {
       const uint32_t i = order[next_arrival++];
       if (need[i] != UINT64_MAX && need[i] <= cfg.hbm_budget_bytes) queue.push_back(i);
+    }
+    // Load shedding: queued requests past their admission deadline leave.
+    if (!queue.empty()) {
+      std::deque<uint32_t> keep;
+      for (uint32_t i : queue) {
+        if (reqs[i].timeout_s > 0 && now - reqs[i].arrival_s > reqs[i].timeout_s) {
+          retire(i, CKT_ERR_TIMEOUT);
+          timed_out++;
+        } else {
+          keep.push_back(i);
+        }
+      }
+      queue.swap(keep);
     }
     // Admission, first come first served.
     while (!queue.empty()) {
@@ -371,6 +384,7 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
     report->latency_max_s = latencies.empty() ? 0 : *std::max_element(latencies.begin(), latencies.end());
     report->completed = (int32_t)latencies.size();
     report->rejected = (int32_t)rejected;
+    report->timed_out = (int32_t)timed_out;
     report->iterations = iterations;
     report->max_resident = max_resident;
   }

@@ -56,6 +56,19 @@ def test_served_results_equal_alone(C, policy, mode):
     s.close()
 
 
+def test_server_sheds_requests_past_their_admission_deadline(C):
+    # One slot: the first request occupies it for 20 loops; the others, all
+    # arriving at t=0 with a 1 ms admission deadline, time out in the queue.
+    s = C.Server(0, max_active=1, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+    reqs = [C.Request(C.GenParams(4, 1024, 4096, 95, 10 + i, C.PROFILE_STABLE), 20, 0.0,
+                      0.001 if i else 0.0) for i in range(4)]
+    results, rep = s.run_trace(reqs)
+    assert results[0]["status"] == 0
+    assert [r["status"] for r in results[1:]] == [11, 11, 11]  # CKT_ERR_TIMEOUT
+    assert rep["timed_out"] == 3 and rep["completed"] == 1
+    s.close()
+
+
 def test_server_rejects_oversized_and_invalid(C):
     s = C.Server(0, budget_bytes=C.footprint_bytes(4096, 16384, 4), policy=C.POLICY_FAST)
     reqs = [C.Request(C.GenParams(4, 1024, 4096, 95, 1, C.PROFILE_STABLE), 1, 0.0),
