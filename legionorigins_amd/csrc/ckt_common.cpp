@@ -36,9 +36,10 @@ static int finish(hipError_t launched, hipStream_t s, const char* what) {
 }
 
 // Device counter for bad-record counts of the accessor-form kernels; one per
-// device, allocated on first use.
+// host thread and device, allocated on first use (so concurrent callers on
+// different threads never share one).
 static unsigned long long* bad_counter(int* status) {
-  static unsigned long long* ptrs[64] = {nullptr};
+  static thread_local unsigned long long* ptrs[64] = {nullptr};
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || dev < 0 || dev >= 64) {
