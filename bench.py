@@ -135,15 +135,21 @@ def run_batched(C, args):
     rep = b.run(best)
     b.close()
     log(f"[batched] {rep}")
-    # Sequential plain launches: each circuit alone, (a) one launch per piece
-    # per task with a wait after each (how the reference runtime runs GPU
-    # point tasks), (b) one launch per task over all pieces, graph-captured.
-    seq_piece = seq_task = 0.0
+    # Sequential launches, each circuit alone:
+    #  (a) one launch per piece per task with a wait after each (how the
+    #      reference runtime runs GPU point tasks);
+    #  (b) plain launches: one per task over all pieces, no fusion, no graph;
+    #  (c) this library's own loop driver: fused epilogue, graph-captured.
+    seq_piece = seq_plain = seq_task = 0.0
     ctx = C.Context(0)
     for i, s in enumerate(shapes):
         ctx.generate(s)
+        ctx.set_option(C.OPT_FUSE_DSC, 0)
         seq_piece += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
                                    per_piece=True)["elapsed_s"]
+        seq_plain += ctx.run_loops(loops, policy=C.POLICY_FAST,
+                                   mode=C.REDUCE_ATOMIC)["elapsed_s"]
+        ctx.set_option(C.OPT_FUSE_DSC, 1)
         seq_task += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
                                   use_graph=True)["elapsed_s"]
         if i % 64 == 63:
@@ -162,9 +168,11 @@ def run_batched(C, args):
                                     "waves_per_simd": best.waves_per_simd}},
            "batched_elapsed_s": rep["elapsed_s"],
            "sequential_per_piece": {"elapsed_s": seq_piece, "wire_updates_per_s": wu / seq_piece},
-           "sequential_per_task_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
+           "sequential_plain": {"elapsed_s": seq_plain, "wire_updates_per_s": wu / seq_plain},
+           "sequential_fused_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
            "speedup_vs_sequential_per_piece": seq_piece / rep["elapsed_s"],
-           "speedup_vs_sequential_per_task_graph": seq_task / rep["elapsed_s"]}
+           "speedup_vs_sequential_plain": seq_plain / rep["elapsed_s"],
+           "speedup_vs_sequential_fused_graph": seq_task / rep["elapsed_s"]}
     print(json.dumps(out))
 
 

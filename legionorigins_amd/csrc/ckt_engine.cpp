@@ -488,8 +488,8 @@ int Engine::upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n,
 }
 
 enum TileKind { TK_WIRES = 1, TK_NODES = 2, TK_CSR = 3, TK_PIECES = 4 };
-// update_voltages: 256 lanes x 4 nodes per tile.
-constexpr int kUpvTile = 1024;
+// update_voltages: 256 lanes x 8 nodes per tile.
+constexpr int kUpvTile = 2048;
 
 int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out) {
   uint64_t key = fnv(1469598103934665603ull, &kind, sizeof(kind));

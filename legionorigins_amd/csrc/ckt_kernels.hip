@@ -90,7 +90,10 @@ struct Relax<CKT_POLICY_RECIP> {
 //   I_0 <- K_0 + (d - g) I_0 - d I_1,  I_9 <- K_9 + (d - g) I_9 - d I_8
 // with d = h*b and K_i = h*(V0 difference across segment i) + g*I0_i.
 // The first step uses the seeded voltages (and its inductive term is zero
-// because the running current still equals I0).  28 FMAs per step.
+// because the running current still equals I0).  28 VALU per step: for the
+// eight interior currents one add (I_{i-1} + I_{i+1}, two register operands)
+// and two FMAs, for the two end currents two FMAs.  The add form issues
+// faster than a third three-operand FMA.
 template <>
 struct Relax<CKT_POLICY_FAST> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
@@ -118,7 +121,7 @@ struct Relax<CKT_POLICY_FAST> {
     float y[kSeg];                                                                 \
     y[0] = fmaf(a_end, x[0], fmaf(m, x[1], K[0]));                                 \
     _Pragma("unroll") for (int i = 1; i < kSeg - 1; i++) y[i] =                    \
-        fmaf(a_mid, x[i], fmaf(m, x[i - 1], fmaf(m, x[i + 1], K[i])));             \
+        fmaf(m, x[i - 1] + x[i + 1], fmaf(a_mid, x[i], K[i]));                     \
     y[kSeg - 1] = fmaf(a_end, x[kSeg - 1], fmaf(m, x[kSeg - 2], K[kSeg - 1]));     \
     _Pragma("unroll") for (int i = 0; i < kSeg; i++) x[i] = y[i];                  \
   }
@@ -308,38 +311,45 @@ __device__ __forceinline__ float upv_one(float v, float q, float C, float lk) {
   return v * (1.f - lk);
 }
 
-// Tiles of up to 4 * 256 nodes: each lane updates 4 consecutive nodes with
-// 128-bit loads and stores when the tile starts on a 16-byte boundary (node
-// arrays start 256-byte aligned), one node at a time otherwise.
+// Tiles of up to 2048 nodes: each lane updates two groups of 4 consecutive
+// nodes, the groups interleaved across the wave so that every 128-bit access
+// of a wave is contiguous.  Aligned groups (tile start on a 16-byte boundary;
+// node arrays start 256-byte aligned) use nontemporal 128-bit loads and
+// stores -- the data is streamed once per loop -- and ragged ones go one node
+// at a time.
 __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__ circuits,
                                                   const Tile* __restrict__ tiles) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) v4f gv4f;
   const Tile t = tiles[blockIdx.x];
-  const uint32_t k0 = 4u * threadIdx.x;
-  if (k0 >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
-  const uint32_t n = t.first + k0;
   gfloat* charge = (gfloat*)c.charge;
   gfloat* voltage = (gfloat*)c.voltage;
   gfloat* ncap = (gfloat*)c.ncap;
   gfloat* leak = (gfloat*)c.leak;
-  if ((t.first & 3u) == 0 && k0 + 4 <= t.count) {
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    typedef __attribute__((address_space(1))) v4f gv4f;
-    const v4f q = *(const gv4f*)(charge + n);
-    const v4f C = *(const gv4f*)(ncap + n);
-    const v4f lk = *(const gv4f*)(leak + n);
-    v4f v = *(gv4f*)(voltage + n);
-    v.x = upv_one(v.x, q.x, C.x, lk.x);
-    v.y = upv_one(v.y, q.y, C.y, lk.y);
-    v.z = upv_one(v.z, q.z, C.z, lk.z);
-    v.w = upv_one(v.w, q.w, C.w, lk.w);
-    *(gv4f*)(voltage + n) = v;
-    *(gv4f*)(charge + n) = (v4f){0.f, 0.f, 0.f, 0.f};
-  } else {
-    const uint32_t end = t.count - k0 < 4u ? t.count - k0 : 4u;
-    for (uint32_t k = 0; k < end; k++) {
-      voltage[n + k] = upv_one(voltage[n + k], charge[n + k], ncap[n + k], leak[n + k]);
-      charge[n + k] = 0.f;
+  const bool aligned = (t.first & 3u) == 0;
+#pragma unroll
+  for (uint32_t g = 0; g < 2; g++) {
+    const uint32_t k0 = 4u * (g * 256u + threadIdx.x);
+    if (k0 >= t.count) continue;
+    const uint32_t n = t.first + k0;
+    if (aligned && k0 + 4 <= t.count) {
+      const v4f q = __builtin_nontemporal_load((const gv4f*)(charge + n));
+      const v4f C = __builtin_nontemporal_load((const gv4f*)(ncap + n));
+      const v4f lk = __builtin_nontemporal_load((const gv4f*)(leak + n));
+      v4f v = __builtin_nontemporal_load((const gv4f*)(voltage + n));
+      v.x = upv_one(v.x, q.x, C.x, lk.x);
+      v.y = upv_one(v.y, q.y, C.y, lk.y);
+      v.z = upv_one(v.z, q.z, C.z, lk.z);
+      v.w = upv_one(v.w, q.w, C.w, lk.w);
+      __builtin_nontemporal_store(v, (gv4f*)(voltage + n));
+      __builtin_nontemporal_store((v4f){0.f, 0.f, 0.f, 0.f}, (gv4f*)(charge + n));
+    } else {
+      const uint32_t end = t.count - k0 < 4u ? t.count - k0 : 4u;
+      for (uint32_t k = 0; k < end; k++) {
+        voltage[n + k] = upv_one(voltage[n + k], charge[n + k], ncap[n + k], leak[n + k]);
+        charge[n + k] = 0.f;
+      }
     }
   }
 }
