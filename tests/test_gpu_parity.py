@@ -236,6 +236,20 @@ def test_accessor_form_dropins(C, O):
         C.calc_new_currents_gpu(c.pieces[0], dw, pvt, owned, ghost)
 
 
+@pytest.mark.parametrize("flag", [1, 2], ids=["recip", "fast"])
+def test_accessor_form_fast_policies(C, O, flag):
+    c = O.generate(4, 256, 1024, 70, 32, O.PROFILE_STABLE)
+    dw = C.DeviceArray(c.wires)
+    pvt, owned, ghost = (C.DeviceArray(c.nodes) for _ in range(3))
+    ref = c.copy()
+    for p in c.pieces:
+        C.calc_new_currents_gpu(p, dw, pvt, owned, ghost, flag=flag)
+        O.calc_new_currents(ref, p)
+    got = dw.download()
+    assert max_wire_rel_err(got["current"], ref.wires["current"]) <= TOL
+    assert max_wire_rel_err(got["voltage"], ref.wires["voltage"]) <= TOL
+
+
 def test_sanity_check_on_context(C):
     with C.Context(0) as ctx:
         ctx.generate(C.GenParams(**{"num_pieces": 8, "nodes_per_piece": 512,
