@@ -57,6 +57,8 @@ def parse():
                     help="served: trace length of the batched-vs-sequential comparison")
     ap.add_argument("--upv-sweep", action="store_true",
                     help="update_voltages HBM bandwidth at growing node counts")
+    ap.add_argument("--gen", action="store_true",
+                    help="circuit generation: host generator + upload vs the device generator")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run only the plain copy kernel over BYTES (profiler calibration)")
     return ap.parse_args()
@@ -271,6 +273,30 @@ def run_upv_sweep(C, args):
                       "algorithmic_bytes_per_node": 24, "rows": rows}))
 
 
+def run_gen(C, args):
+    rows = []
+    for P, npp, wpp in ((64, 16384, 65536), (1024, 16384, 65536)):
+        gp = C.GenParams(P, npp, wpp, 95, 12345, C.PROFILE_REFERENCE)
+        row = {"pieces": P, "nodes": P * npp, "wires": P * wpp}
+        for name, fn in (("host", "generate"), ("device", "generate_device")):
+            ctx = C.Context(0)
+            getattr(ctx, fn)(gp)  # first call pays one-time costs
+            t0 = time.perf_counter()
+            getattr(ctx, fn)(gp)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            row[name + "_s"] = dt
+            row[name + "_wires_per_s"] = P * wpp / dt
+            ctx.close()
+        row["speedup"] = row["host_s"] / row["device_s"]
+        rows.append(row)
+        log(f"[gen] {row}")
+    print(json.dumps({"metric": "circuit generation (load_circuit) throughput", "unit": "wires/s",
+                      "host": "generator on one host thread + slab upload",
+                      "device": "start-state chain on the host, draws/classification/index on the GPU",
+                      "rows": rows}))
+
+
 def run_copy_calibration(C, nbytes):
     import numpy as np
     src = C.DeviceArray(np.zeros(nbytes // 4, np.float32))
@@ -296,6 +322,8 @@ def main():
         return run_served(C, args)
     if args.upv_sweep:
         return run_upv_sweep(C, args)
+    if args.gen:
+        return run_gen(C, args)
 
     policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
     mode = {"atomic": C.REDUCE_ATOMIC, "csr": C.REDUCE_CSR,

@@ -223,9 +223,23 @@ int ckt_ctx_load(ckt_ctx_t* ctx, const ckt_node_t* nodes, uint32_t num_nodes,
                  const ckt_piece_t* pieces, uint32_t num_pieces);
 /* Generate (ckt_generate) and load in one call. */
 int ckt_ctx_generate(ckt_ctx_t* ctx, const ckt_gen_params_t* params);
+/* Same circuit, bit for bit, generated in HBM (load_circuit,
+ * circuit.cc:1071-1288, without the host image or its upload): the host
+ * walks the drand48 chain once to find each wire's first draw, the device
+ * draws every node and wire, marks shared nodes, derives the locations and
+ * sorts the node->wire index. */
+int ckt_ctx_generate_device(ckt_ctx_t* ctx, const ckt_gen_params_t* params);
 /* Download the current state back into the reference layouts (host memory;
  * either pointer may be NULL).  Waits for the stream. */
 int ckt_ctx_store(ckt_ctx_t* ctx, ckt_node_t* nodes, ckt_wire_t* wires);
+/* Download the loaded circuit's topology (any pointer may be NULL): the
+ * locator (1 private, 0 shared; circuit.cc:1210-1220) as num_nodes ints, and
+ * the node->wire index the CSR reduction modes use: csr_row[num_nodes+1]
+ * offsets into csr_ent[2*num_wires], entry = wire << 1 | end (0 in, 1 out),
+ * each node's entries in (wire, in-before-out) order.  Waits for the
+ * stream. */
+int ckt_ctx_store_topology(ckt_ctx_t* ctx, int32_t* locator, uint32_t* csr_row,
+                           uint32_t* csr_ent);
 int ckt_ctx_sizes(ckt_ctx_t* ctx, uint32_t* num_nodes, uint32_t* num_wires,
                   uint32_t* num_pieces);
 int ckt_ctx_pieces(ckt_ctx_t* ctx, ckt_piece_t* pieces);
@@ -315,6 +329,10 @@ int ckt_batch_destroy(ckt_batch_t* b);
  * footprint does not fit in the remaining budget. */
 int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int32_t num_loops,
                             uint32_t* id);
+/* ckt_batch_add_generated with the circuit generated in HBM (as
+ * ckt_ctx_generate_device). */
+int ckt_batch_add_generated_device(ckt_batch_t* b, const ckt_gen_params_t* params,
+                                   int32_t num_loops, uint32_t* id);
 /* Generate `n` circuits on `threads` host threads and add them in order;
  * ids[i] receives the id of params[i].  Stops at the first failure. */
 int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,

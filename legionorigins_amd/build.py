@@ -19,8 +19,8 @@ DRIVER = os.path.join(HERE, "ckt_sim")
 BUILD_DIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
-SOURCES = ["ckt_kernels.hip", "ckt_common.cpp", "ckt_generator.cpp", "ckt_engine.cpp",
-           "ckt_ctx.cpp", "ckt_batch.cpp", "ckt_server.cpp"]
+SOURCES = ["ckt_kernels.hip", "ckt_devgen.hip", "ckt_common.cpp", "ckt_generator.cpp",
+           "ckt_engine.cpp", "ckt_ctx.cpp", "ckt_batch.cpp", "ckt_server.cpp"]
 
 COMMON_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                 "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

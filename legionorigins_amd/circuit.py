@@ -158,6 +158,8 @@ def _declare(L):
         "ckt_ctx_synchronize": ([vp], i32),
         "ckt_ctx_load": ([vp, vp, u32, vp, u32, vp, vp, u32], i32),
         "ckt_ctx_generate": ([vp, P(_GenParams)], i32),
+        "ckt_ctx_generate_device": ([vp, P(_GenParams)], i32),
+        "ckt_ctx_store_topology": ([vp, vp, vp, vp], i32),
         "ckt_ctx_store": ([vp, vp, vp], i32),
         "ckt_ctx_sizes": ([vp, P(u32), P(u32), P(u32)], i32),
         "ckt_ctx_pieces": ([vp, vp], i32),
@@ -177,6 +179,7 @@ def _declare(L):
         "ckt_batch_set_timing": ([vp, i32], i32),
         "ckt_batch_kernel_time": ([vp, i32, P(ctypes.c_double), P(u64)], i32),
         "ckt_batch_add_generated": ([vp, P(_GenParams), i32, P(u32)], i32),
+        "ckt_batch_add_generated_device": ([vp, P(_GenParams), i32, P(u32)], i32),
         "ckt_batch_add_generated_many": ([vp, P(_GenParams), P(ctypes.c_int32), u32, i32,
                                           P(u32)], i32),
         "ckt_batch_add": ([vp, vp, u32, vp, u32, vp, vp, u32, i32, P(u32)], i32),
@@ -375,6 +378,20 @@ class Context:
         cp = params.c()
         _check(lib().ckt_ctx_generate(self.h, ctypes.byref(cp)))
 
+    def generate_device(self, params: GenParams):
+        """The circuit generate() builds, bit for bit, generated in HBM."""
+        cp = params.c()
+        _check(lib().ckt_ctx_generate_device(self.h, ctypes.byref(cp)))
+
+    def store_topology(self):
+        """(locator, csr_row, csr_ent) of the loaded circuit."""
+        n, w, _ = self.sizes()
+        loc = np.zeros(n, np.int32)
+        row = np.zeros(n + 1, np.uint32)
+        ent = np.zeros(2 * w, np.uint32)
+        _check(lib().ckt_ctx_store_topology(self.h, _p(loc), _p(row), _p(ent)))
+        return loc, row, ent
+
     def sizes(self):
         n, w, p = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().ckt_ctx_sizes(self.h, ctypes.byref(n), ctypes.byref(w), ctypes.byref(p)))
@@ -473,6 +490,13 @@ class Batch:
         cid = ctypes.c_uint32()
         cp = params.c()
         _check(lib().ckt_batch_add_generated(self.h, ctypes.byref(cp), loops, ctypes.byref(cid)))
+        return cid.value
+
+    def add_generated_device(self, params: GenParams, loops):
+        cid = ctypes.c_uint32()
+        cp = params.c()
+        _check(lib().ckt_batch_add_generated_device(self.h, ctypes.byref(cp), loops,
+                                                    ctypes.byref(cid)))
         return cid.value
 
     def add_generated_many(self, params, loops, threads=8):

@@ -108,6 +108,14 @@ int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int3
   return b->eng.add(img, num_loops, id);
 }
 
+int ckt_batch_add_generated_device(ckt_batch_t* b, const ckt_gen_params_t* params,
+                                   int32_t num_loops, uint32_t* id) {
+  int st = check(b);
+  if (st) return st;
+  if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
+  return b->eng.add_generated_on_device(params, num_loops, id);
+}
+
 int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,
                                  const int32_t* num_loops, uint32_t n, int32_t threads,
                                  uint32_t* ids) {

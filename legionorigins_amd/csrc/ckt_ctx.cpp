@@ -100,10 +100,30 @@ int ckt_ctx_generate(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
   return install(ctx, img);
 }
 
+int ckt_ctx_generate_device(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
+  if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
+  int st = gen_validate(params, nullptr, nullptr, nullptr);
+  if (st) return st;
+  if (ctx->loaded) {
+    if ((st = ctx->eng.remove(ctx->id))) return st;
+    ctx->loaded = false;
+  }
+  if ((st = ctx->eng.add_generated_on_device(params, 0, &ctx->id))) return st;
+  ctx->loaded = true;
+  return CKT_OK;
+}
+
 int ckt_ctx_store(ckt_ctx_t* ctx, ckt_node_t* nodes, ckt_wire_t* wires) {
   int st = check_ctx(ctx);
   if (st) return st;
   return ctx->eng.store(ctx->id, nodes, wires);
+}
+
+int ckt_ctx_store_topology(ckt_ctx_t* ctx, int32_t* locator, uint32_t* csr_row,
+                           uint32_t* csr_ent) {
+  int st = check_ctx(ctx);
+  if (st) return st;
+  return ctx->eng.store_topology(ctx->id, locator, csr_row, csr_ent);
 }
 
 int ckt_ctx_sizes(ckt_ctx_t* ctx, uint32_t* num_nodes, uint32_t* num_wires, uint32_t* num_pieces) {

@@ -275,17 +275,44 @@ int Engine::add(SlabImage& img, int32_t loops, uint32_t* id) {
     (void)hipFree(slab);
     return hip_fail(e, "hipMemcpy(circuit upload)");
   }
+  attach(slab, img.lay, img.N, img.W, img.P, img.pieces, loops, id);
+  return CKT_OK;
+}
+
+int Engine::add_generated_on_device(const ckt_gen_params_t* params, int32_t loops, uint32_t* id) {
+  uint32_t N, W, P;
+  int st = gen_validate(params, &N, &W, &P);
+  if (st) return st;
+  CKT_HIP(hipSetDevice(device_));
+  const SlabLayout lay = SlabLayout::make(N, W, P);
+  if (budget_ && in_use_ + lay.total > budget_)
+    return fail(CKT_ERR_BUDGET, "circuit needs " + std::to_string(lay.total) +
+                                    " bytes; budget has " + std::to_string(budget_ - in_use_) + " left");
+  void* slab = nullptr;
+  hipError_t e = hipMalloc(&slab, lay.total);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(circuit)");
+  std::vector<ckt_piece_t> pieces;
+  if ((st = devgen_build(params, lay, slab, stream_, pieces))) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(slab);
+    return st;
+  }
+  attach(slab, lay, N, W, P, pieces, loops, id);
+  return CKT_OK;
+}
+
+void Engine::attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uint32_t P,
+                    const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id) {
   Circuit* c = new Circuit();
   c->id = next_id_++;
-  c->num_nodes = img.N;
-  c->num_wires = img.W;
-  c->num_pieces = img.P;
-  c->lay = img.lay;
+  c->num_nodes = N;
+  c->num_wires = W;
+  c->num_pieces = P;
+  c->lay = L;
   c->slab = slab;
-  c->pieces = img.pieces;
+  c->pieces = pieces;
   c->loops_left = loops;
   uint8_t* b = (uint8_t*)slab;
-  const SlabLayout& L = img.lay;
   DevCircuit& d = c->dev;
   d.cur = (float*)(b + L.cur);
   d.vol = (float*)(b + L.vol);
@@ -304,15 +331,15 @@ int Engine::add(SlabImage& img, int32_t loops, uint32_t* id) {
   d.csr_row = (uint32_t*)(b + L.csr_row);
   d.csr_ent = (uint32_t*)(b + L.csr_ent);
   d.pieces = (ckt_piece_t*)(b + L.pieces);
-  d.num_wires = img.W;
+  d.num_wires = W;
   d.ws = L.ws;
-  d.num_nodes = img.N;
-  d.num_pieces = img.P;
-  for (uint32_t k = 0; k < img.P; k++) c->piece_index.emplace(piece_key(img.pieces[k]), k);
+  d.num_nodes = N;
+  d.num_pieces = P;
+  for (uint32_t k = 0; k < P; k++) c->piece_index.emplace(piece_key(pieces[k]), k);
   // Wires partitioned by the pieces -> the CSR modes apply; node ranges
   // disjoint -> the LDS-privatised reduction applies.
   {
-    std::vector<ckt_piece_t> ps = img.pieces;
+    std::vector<ckt_piece_t> ps = pieces;
     std::sort(ps.begin(), ps.end(), [](const ckt_piece_t& a, const ckt_piece_t& x) {
       return a.first_wire < x.first_wire;
     });
@@ -323,7 +350,7 @@ int Engine::add(SlabImage& img, int32_t loops, uint32_t* id) {
       if (p.first_wire != at) ok = false;
       at = (uint64_t)p.first_wire + p.num_wires;
     }
-    c->csr_ok = ok && at == img.W;
+    c->csr_ok = ok && at == W;
     std::sort(ps.begin(), ps.end(), [](const ckt_piece_t& a, const ckt_piece_t& x) {
       return a.first_node < x.first_node;
     });
@@ -338,11 +365,10 @@ int Engine::add(SlabImage& img, int32_t loops, uint32_t* id) {
   }
   slots_.push_back(c);
   by_id_[c->id] = c;
-  in_use_ += img.lay.total;
+  in_use_ += L.total;
   table_dirty_ = true;
   generation_++;
   if (id) *id = c->id;
-  return CKT_OK;
 }
 
 int Engine::remove(uint32_t id) {
@@ -431,6 +457,25 @@ int Engine::store(uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires) {
     for (uint32_t i = 0; i < c->num_nodes; i++)
       nodes[i] = ckt_node_t{charge[i], voltage[i], ncap[i], leak[i]};
   }
+  return CKT_OK;
+}
+
+int Engine::store_topology(uint32_t id, int32_t* locator, uint32_t* csr_row, uint32_t* csr_ent) {
+  Circuit* c = find(id);
+  if (!c) return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(id));
+  CKT_HIP(hipSetDevice(device_));
+  CKT_HIP(hipStreamSynchronize(stream_));
+  const uint8_t* b = (const uint8_t*)c->slab;
+  const SlabLayout& L = c->lay;
+  if (locator) {
+    std::vector<uint8_t> loc(c->num_nodes);
+    CKT_HIP(hipMemcpy(loc.data(), b + L.locator, c->num_nodes, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < c->num_nodes; i++) locator[i] = loc[i];
+  }
+  if (csr_row)
+    CKT_HIP(hipMemcpy(csr_row, b + L.csr_row, ((size_t)c->num_nodes + 1) * 4, hipMemcpyDeviceToHost));
+  if (csr_ent && c->num_wires)
+    CKT_HIP(hipMemcpy(csr_ent, b + L.csr_ent, (size_t)c->num_wires * 8, hipMemcpyDeviceToHost));
   return CKT_OK;
 }
 

@@ -45,6 +45,12 @@ int image_from_aos(SlabImage& img, const ckt_node_t* nodes, uint32_t N, const ck
                    uint32_t W, const int32_t* locator, const ckt_piece_t* pieces, uint32_t P);
 int image_from_gen(SlabImage& img, const ckt_gen_params_t* params);
 
+// Generates the circuit `params` describes straight into a device slab of
+// layout `L` (ckt_devgen.hip), bit-identical to image_from_gen; fills
+// `pieces`.  Synchronises `s` before returning.
+int devgen_build(const ckt_gen_params_t* params, const SlabLayout& L, void* slab, hipStream_t s,
+                 std::vector<ckt_piece_t>& pieces);
+
 struct Circuit {
   uint32_t id = 0;
   uint32_t num_nodes = 0, num_wires = 0, num_pieces = 0;
@@ -86,9 +92,12 @@ class Engine {
 
   // Circuits.
   int add(SlabImage& img, int32_t loops, uint32_t* id);
+  // Generates the circuit on the device (no host image, no upload).
+  int add_generated_on_device(const ckt_gen_params_t* params, int32_t loops, uint32_t* id);
   int remove(uint32_t id);
   Circuit* find(uint32_t id);
   int store(uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
+  int store_topology(uint32_t id, int32_t* locator, uint32_t* csr_row, uint32_t* csr_ent);
   int download_voltages(uint32_t id, std::vector<float>& v);
   size_t count() const { return slots_.size(); }
   uint64_t bytes_in_use() const { return in_use_; }
@@ -132,6 +141,8 @@ class Engine {
     int task;
     hipEvent_t a, b;
   };
+  void attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uint32_t P,
+              const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id);
   int ensure_table();
   int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
   int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
