@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--policy", default="fast", choices=["exact", "recip", "fast"])
     ap.add_argument("--mode", default="atomic", choices=["atomic", "csr", "kahan64"])
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--profile", default="reference", choices=["reference", "stable"],
+                    help="generator profile of the timed circuit (stable: finite values)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true",
@@ -330,7 +332,8 @@ def main():
             "kahan64": C.REDUCE_CSR_KAHAN64}[args.mode]
     wl = dict(WORKLOADS[args.workload])
     wl.pop("loops")
-    params = C.GenParams(**wl, profile=C.PROFILE_REFERENCE)
+    params = C.GenParams(**wl, profile={"reference": C.PROFILE_REFERENCE,
+                                        "stable": C.PROFILE_STABLE}[args.profile])
     info = C.device_info(0)
     log(f"[bench] device {info}")
     ctx = C.Context(0)
@@ -362,7 +365,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (load_circuit generator, seed 12345)",
+        "data": f"synthetic (load_circuit generator, seed 12345, {args.profile} profile)",
         "config": {"workload": f"{P} pieces x {N // P} nodes x {W // P} wires/piece "
                                f"({W} wires, {N} nodes), 95% in-piece, seed 12345, "
                                f"WIRE_SEGMENTS=10, 10000 inner steps/loop, FP32, SoA",
@@ -418,6 +421,20 @@ def main():
                                "distribute_charge_ms": k4[C.DISTRIBUTE_CHARGE][0]}
         detail["distribute_charge"] = red
         log(f"[bench] reductions: {red}")
+        # The same workload in the stable profile: the reference parameters
+        # overflow to inf/NaN within ~10 inner steps, the stable ones stay
+        # finite, so this row shows the rate on finite data.
+        ctx_s = C.Context(0)
+        ctx_s.generate_device(C.GenParams(**wl, profile=C.PROFILE_STABLE))
+        ctx_s.set_timing(True)
+        ctx_s.run_loops(1, policy=policy, mode=mode)
+        r5, w5, k5 = time_loops(C, ctx_s, 4, policy, mode)
+        ms = k5[C.CALC_NEW_CURRENTS][0]
+        detail["stable_profile"] = {
+            "wire_updates_per_s": W * 4 / w5, "cnc_ms": ms,
+            "tflops_algorithmic": FLOPS_PER_WIRE_STEP * C.STEPS * W / ms / 1e9}
+        log(f"[bench] stable profile: {detail['stable_profile']}")
+        ctx_s.close()
     out["detail"] = detail
     ctx.close()
     if not args.no_cpu:

@@ -102,7 +102,12 @@ struct Relax<CKT_POLICY_FAST> {
     const float dt = CKT_DT_F;
     const float h = 1.0f / R, g = (L / dt) / R, b = dt / C;
     const float d = h * b;
-    const float a_end = d - g, a_mid = 2.0f * d - g, m = -d;
+    const float a_end = d - g, a_mid = 2.0f * d - g;
+    // m gets a register of its own: folded into the FMAs as a negation
+    // modifier it would force every outer FMA into the 8-byte three-source
+    // encoding instead of the 4-byte accumulate form.
+    float m = -d;
+    asm volatile("" : "+v"(m));
     float K[kSeg], x[kSeg];
     {
       float dv[kSeg];
