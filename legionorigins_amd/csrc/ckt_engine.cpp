@@ -535,6 +535,11 @@ int Engine::upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n,
 enum TileKind { TK_WIRES = 1, TK_NODES = 2, TK_CSR = 3, TK_PIECES = 4 };
 // update_voltages: 256 lanes x 8 nodes per tile.
 constexpr int kUpvTile = 2048;
+// Node bytes update_voltages moves per node (4 arrays read, 2 written) and the
+// MI355X last-level cache (256 MB of Infinity Cache): a launch over more node
+// bytes than that streams them with nontemporal accesses.
+constexpr uint64_t kUpvBytesPerNode = 24;
+constexpr uint64_t kLastLevelCacheBytes = 256ull << 20;
 
 int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out) {
   uint64_t key = fnv(1469598103934665603ull, &kind, sizeof(kind));
@@ -732,7 +737,10 @@ int Engine::upv(const std::vector<PieceRef>& refs) {
   if ((st = tiles_for(TK_NODES, refs, kUpvTile, &t))) return st;
   hipEvent_t a;
   if ((st = timed_begin(CKT_UPDATE_VOLTAGES, &a))) return st;
-  hipError_t e = launch_upv(d_table_, (const Tile*)t.p, t.n, 256, stream_);
+  uint64_t nodes = 0;
+  for (const PieceRef& r : refs) nodes += slots_[r.slot]->pieces[r.piece].num_nodes;
+  const bool nontemporal = nodes * kUpvBytesPerNode > kLastLevelCacheBytes;
+  hipError_t e = launch_upv(d_table_, (const Tile*)t.p, t.n, 256, nontemporal, stream_);
   if (e != hipSuccess) return hip_fail(e, "update_voltages launch");
   return timed_end(CKT_UPDATE_VOLTAGES, a);
 }

@@ -67,8 +67,10 @@ hipError_t launch_dsc_lds(const DevCircuit* circuits, const PieceTile* ptiles, u
                           size_t lds_bytes, hipStream_t s);
 hipError_t launch_dsc_csr(int kahan, const DevCircuit* circuits, const Tile* node_tiles,
                           uint32_t ntiles, int block, hipStream_t s);
+// nontemporal: streaming accesses for node arrays larger than the last-level
+// cache.
 hipError_t launch_upv(const DevCircuit* circuits, const Tile* node_tiles, uint32_t ntiles,
-                      int block, hipStream_t s);
+                      int block, bool nontemporal, hipStream_t s);
 hipError_t launch_sanity(const DevCircuit* circuits, const Tile* tiles, uint32_t ntiles,
                          int block, unsigned long long* bad, hipStream_t s);
 hipError_t launch_nop(hipStream_t s);

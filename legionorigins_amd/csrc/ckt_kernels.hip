@@ -316,16 +316,32 @@ __device__ __forceinline__ float upv_one(float v, float q, float C, float lk) {
   return v * (1.f - lk);
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4f gv4f;
+
+template <bool NT>
+__device__ __forceinline__ v4f load4(const gv4f* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <bool NT>
+__device__ __forceinline__ void store4(v4f v, gv4f* p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Tiles of up to 2048 nodes: each lane updates two groups of 4 consecutive
 // nodes, the groups interleaved across the wave so that every 128-bit access
 // of a wave is contiguous.  Aligned groups (tile start on a 16-byte boundary;
-// node arrays start 256-byte aligned) use nontemporal 128-bit loads and
-// stores -- the data is streamed once per loop -- and ragged ones go one node
-// at a time.
+// node arrays start 256-byte aligned) use 128-bit loads and stores, ragged
+// ones go one node at a time.  NT: nontemporal accesses, for launches whose
+// node arrays are larger than the last-level cache (they are streamed once
+// per loop); smaller ones keep ordinary accesses so that a cache-resident
+// working set stays resident.
+template <bool NT>
 __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__ circuits,
                                                   const Tile* __restrict__ tiles) {
-  typedef float v4f __attribute__((ext_vector_type(4)));
-  typedef __attribute__((address_space(1))) v4f gv4f;
   const Tile t = tiles[blockIdx.x];
   const DevCircuit& c = circuits[t.circuit];
   gfloat* charge = (gfloat*)c.charge;
@@ -339,16 +355,16 @@ __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__
     if (k0 >= t.count) continue;
     const uint32_t n = t.first + k0;
     if (aligned && k0 + 4 <= t.count) {
-      const v4f q = __builtin_nontemporal_load((const gv4f*)(charge + n));
-      const v4f C = __builtin_nontemporal_load((const gv4f*)(ncap + n));
-      const v4f lk = __builtin_nontemporal_load((const gv4f*)(leak + n));
-      v4f v = __builtin_nontemporal_load((const gv4f*)(voltage + n));
+      const v4f q = load4<NT>((const gv4f*)(charge + n));
+      const v4f C = load4<NT>((const gv4f*)(ncap + n));
+      const v4f lk = load4<NT>((const gv4f*)(leak + n));
+      v4f v = load4<NT>((const gv4f*)(voltage + n));
       v.x = upv_one(v.x, q.x, C.x, lk.x);
       v.y = upv_one(v.y, q.y, C.y, lk.y);
       v.z = upv_one(v.z, q.z, C.z, lk.z);
       v.w = upv_one(v.w, q.w, C.w, lk.w);
-      __builtin_nontemporal_store(v, (gv4f*)(voltage + n));
-      __builtin_nontemporal_store((v4f){0.f, 0.f, 0.f, 0.f}, (gv4f*)(charge + n));
+      store4<NT>(v, (gv4f*)(voltage + n));
+      store4<NT>((v4f){0.f, 0.f, 0.f, 0.f}, (gv4f*)(charge + n));
     } else {
       const uint32_t end = t.count - k0 < 4u ? t.count - k0 : 4u;
       for (uint32_t k = 0; k < end; k++) {
@@ -609,10 +625,12 @@ hipError_t launch_dsc_csr(int kahan, const DevCircuit* c, const Tile* t, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_upv(const DevCircuit* c, const Tile* t, uint32_t n, int block, hipStream_t s) {
+hipError_t launch_upv(const DevCircuit* c, const Tile* t, uint32_t n, int block, bool nontemporal,
+                      hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (block != 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(upv_kernel, dim3(n), dim3(256), 0, s, c, t);
+  if (nontemporal) hipLaunchKernelGGL(upv_kernel<true>, dim3(n), dim3(256), 0, s, c, t);
+  else hipLaunchKernelGGL(upv_kernel<false>, dim3(n), dim3(256), 0, s, c, t);
   return hipGetLastError();
 }
 
