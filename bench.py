@@ -256,7 +256,7 @@ def run_upv_sweep(C, args):
     for P, npp in ((64, 16384), (256, 65536), (1024, 65536), (2048, 65536)):
         N = P * npp
         ctx = C.Context(0)
-        ctx.generate(C.GenParams(P, npp, 0, 95, 1, C.PROFILE_REFERENCE))
+        ctx.generate_device(C.GenParams(P, npp, 0, 95, 1, C.PROFILE_REFERENCE))
         pieces = ctx.pieces()
         ctx.update_voltages(pieces)
         ctx.synchronize()

@@ -6,6 +6,7 @@
 //           [-pct pct_in_piece] [-s seed] [-sync 0|1|2]
 //           [-policy exact|recip|fast] [-mode atomic|csr|kahan64]
 //           [-profile reference|stable] [-graph] [-per_piece] [-device d]
+//           [-devgen]   (generate the circuit in HBM instead of on the host)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,7 +26,7 @@ int main(int argc, char** argv) {
   int num_loops = 2, num_pieces = 4, nodes_per_piece = 2, wires_per_piece = 4;
   int pct_wire_in_piece = 95, random_seed = 12345, sync = 0;
   int policy = CKT_POLICY_EXACT, mode = CKT_REDUCE_ATOMIC, profile = CKT_PROFILE_REFERENCE;
-  int graph = 0, per_piece = 0, device = 0;
+  int graph = 0, per_piece = 0, device = 0, devgen = 0;
   for (int i = 1; i < argc; i++) {
     auto next_int = [&](int& v) {
       if (i + 1 < argc) v = std::atoi(argv[++i]);
@@ -41,6 +42,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "-device")) next_int(device);
     else if (!std::strcmp(argv[i], "-graph")) graph = 1;
     else if (!std::strcmp(argv[i], "-per_piece")) per_piece = 1;
+    else if (!std::strcmp(argv[i], "-devgen")) devgen = 1;
     else if (!std::strcmp(argv[i], "-policy")) {
       const char* s = next_str();
       policy = !std::strcmp(s, "fast") ? CKT_POLICY_FAST
@@ -61,7 +63,7 @@ int main(int argc, char** argv) {
   check(ckt_ctx_create(device, &ctx), "ckt_ctx_create");
   ckt_gen_params_t gp = {num_pieces, nodes_per_piece, wires_per_piece, pct_wire_in_piece,
                          random_seed, profile};
-  check(ckt_ctx_generate(ctx, &gp), "load_circuit");
+  check(devgen ? ckt_ctx_generate_device(ctx, &gp) : ckt_ctx_generate(ctx, &gp), "load_circuit");
   check(ckt_ctx_set_timing(ctx, 1), "timing");
   std::printf("Starting main simulation loop\n");
   ckt_loop_config_t cfg = {policy, mode, sync, graph, per_piece};
