@@ -419,6 +419,12 @@ def main():
         red["unfused_loop"] = {"wire_updates_per_s": W * 4 / w4,
                                "calc_new_currents_ms": k4[C.CALC_NEW_CURRENTS][0],
                                "distribute_charge_ms": k4[C.DISTRIBUTE_CHARGE][0]}
+        # full loops with the deterministic reductions (their own launch)
+        for name, mv in (("csr_loop", C.REDUCE_CSR), ("kahan64_loop", C.REDUCE_CSR_KAHAN64)):
+            r6, w6, k6 = time_loops(C, ctx, 4, policy, mv)
+            red[name] = {"wire_updates_per_s": W * 4 / w6,
+                         "calc_new_currents_ms": k6[C.CALC_NEW_CURRENTS][0],
+                         "distribute_charge_ms": k6[C.DISTRIBUTE_CHARGE][0]}
         detail["distribute_charge"] = red
         log(f"[bench] reductions: {red}")
         # The same workload in the stable profile: the reference parameters
