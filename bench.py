@@ -204,29 +204,34 @@ def run_served(C, args):
         cmp[name] = {k: r2[k] for k in ("wire_updates_per_s", "elapsed_s", "latency_p50_s",
                                         "latency_p99_s", "completed")}
         log(f"[served] {name}: {cmp[name]}")
-    # Reference-style sequential service: one request at a time in arrival
-    # order, each run with one launch per piece per task and a wait after each
-    # (how the reference runtime executes GPU point tasks).  Service times are
-    # measured on the GPU; queueing is computed from the arrival times.
+    # One request at a time in arrival order, in two launch styles, with
+    # service times measured on the GPU and queueing computed from the
+    # arrival times:
+    #  - plain: one launch per task over all of the request's pieces, no
+    #    fused epilogue, no graph;
+    #  - reference style: one launch per piece per task and a wait after each
+    #    (how the reference runtime executes GPU point tasks).
     import numpy as np
-    ctx = C.Context(0)
-    ctx.set_option(C.OPT_FUSE_DSC, 0)
-    free_at, lat, wu = 0.0, [], 0.0
-    for r in short:
-        ctx.generate(r.gen)
-        svc = ctx.run_loops(r.num_loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
-                            per_piece=True)["elapsed_s"]
-        start = max(r.arrival_s, free_at)
-        free_at = start + svc
-        lat.append(free_at - r.arrival_s)
-        wu += r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops
-    ctx.close()
-    span = free_at - (short[0].arrival_s if short else 0.0)
-    cmp["sequential_reference_style"] = {
-        "wire_updates_per_s": wu / span if span > 0 else 0.0, "elapsed_s": span,
-        "latency_p50_s": float(np.percentile(lat, 50)) if lat else 0.0,
-        "latency_p99_s": float(np.percentile(lat, 99)) if lat else 0.0, "completed": len(lat)}
-    log(f"[served] sequential_reference_style: {cmp['sequential_reference_style']}")
+    for name, per_piece in (("sequential_plain", False), ("sequential_reference_style", True)):
+        ctx = C.Context(0)
+        ctx.set_option(C.OPT_FUSE_DSC, 0)
+        free_at, lat, wu = 0.0, [], 0.0
+        for r in short:
+            ctx.generate(r.gen)
+            svc = ctx.run_loops(r.num_loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+                                per_piece=per_piece)["elapsed_s"]
+            start = max(r.arrival_s, free_at)
+            free_at = start + svc
+            lat.append(free_at - r.arrival_s)
+            wu += r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops
+        ctx.close()
+        span = free_at - (short[0].arrival_s if short else 0.0)
+        cmp[name] = {
+            "wire_updates_per_s": wu / span if span > 0 else 0.0, "elapsed_s": span,
+            "latency_p50_s": float(np.percentile(lat, 50)) if lat else 0.0,
+            "latency_p99_s": float(np.percentile(lat, 99)) if lat else 0.0,
+            "completed": len(lat)}
+        log(f"[served] {name}: {cmp[name]}")
     out = {"metric": "served wire-updates/s with p50/p99 request latency",
            "value": rep["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
            "dtype": "f32", "data": "synthetic (Poisson arrivals, load_circuit generator)",
@@ -246,6 +251,8 @@ def run_served(C, args):
                "seconds": args.compare_seconds, **cmp,
                "speedup": cmp["continuous"]["wire_updates_per_s"]
                / max(cmp["sequential"]["wire_updates_per_s"], 1e-30),
+               "speedup_vs_plain": cmp["continuous"]["wire_updates_per_s"]
+               / max(cmp["sequential_plain"]["wire_updates_per_s"], 1e-30),
                "speedup_vs_reference_style": cmp["continuous"]["wire_updates_per_s"]
                / max(cmp["sequential_reference_style"]["wire_updates_per_s"], 1e-30)}}
     print(json.dumps(out))
