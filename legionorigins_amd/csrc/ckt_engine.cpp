@@ -371,18 +371,29 @@ void Engine::attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uin
   if (id) *id = c->id;
 }
 
-int Engine::remove(uint32_t id) {
-  auto it = by_id_.find(id);
-  if (it == by_id_.end()) return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(id));
+int Engine::remove(uint32_t id) { return remove_many(&id, 1); }
+
+// One stream synchronisation for the whole set.  Cached tile lists and graphs
+// are keyed by the engine generation, which changes here, so entries naming
+// the removed circuits can never be looked up again; they stay until the
+// cache's size bound evicts them (drop_caches waits for the stream first).
+int Engine::remove_many(const uint32_t* ids, size_t n) {
+  for (size_t k = 0; k < n; k++)
+    if (!by_id_.count(ids[k]))
+      return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(ids[k]));
+  if (n == 0) return CKT_OK;
   CKT_HIP(hipSetDevice(device_));
   CKT_HIP(hipStreamSynchronize(stream_));
-  Circuit* c = it->second;
-  drop_caches();
-  CKT_HIP(hipFree(c->slab));
-  in_use_ -= c->lay.total;
-  slots_.erase(std::find(slots_.begin(), slots_.end(), c));
-  by_id_.erase(it);
-  delete c;
+  for (size_t k = 0; k < n; k++) {
+    auto it = by_id_.find(ids[k]);
+    if (it == by_id_.end()) continue;  // listed twice
+    Circuit* c = it->second;
+    CKT_HIP(hipFree(c->slab));
+    in_use_ -= c->lay.total;
+    slots_.erase(std::find(slots_.begin(), slots_.end(), c));
+    by_id_.erase(it);
+    delete c;
+  }
   table_dirty_ = true;
   generation_++;
   return CKT_OK;

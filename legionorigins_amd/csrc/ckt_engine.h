@@ -95,6 +95,7 @@ class Engine {
   // Generates the circuit on the device (no host image, no upload).
   int add_generated_on_device(const ckt_gen_params_t* params, int32_t loops, uint32_t* id);
   int remove(uint32_t id);
+  int remove_many(const uint32_t* ids, size_t n);
   Circuit* find(uint32_t id);
   int store(uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
   int store_topology(uint32_t id, int32_t* locator, uint32_t* csr_row, uint32_t* csr_ent);

@@ -359,7 +359,12 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
         s->kept_wires[ri].resize(c->num_wires);
         if ((st = E.store(id, s->kept_nodes[ri].data(), s->kept_wires[ri].data()))) break;
       }
-      if ((st = E.remove(id))) break;
+    }
+    if (st) break;
+    // One removal for the whole set: a single stream synchronisation.
+    if ((st = E.remove_many(done.data(), done.size()))) break;
+    for (uint32_t id : done) {
+      const uint32_t ri = request_of_circuit[id];
       reserved -= need[ri];
       admitted_wires -= wires_of[ri];
       admitted--;
@@ -370,7 +375,7 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
   // Leave nothing resident, whatever happened.
   std::vector<uint32_t> left;
   for (Circuit* c : E.slots()) left.push_back(c->id);
-  for (uint32_t id : left) (void)E.remove(id);
+  (void)E.remove_many(left.data(), left.size());
   if (st) return st;
   if (report) {
     report->elapsed_s = elapsed;
