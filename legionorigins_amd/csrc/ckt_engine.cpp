@@ -236,6 +236,8 @@ Engine::~Engine() {
   if (d_table_) (void)hipFree(d_table_);
   if (d_counters_) (void)hipFree(d_counters_);
   if (d_bad_) (void)hipFree(d_bad_);
+  if (devgen_scratch_) (void)hipFree(devgen_scratch_);
+  for (const PinnedChunk& c : staging_) (void)hipHostFree(c.p);
   for (auto& p : pending_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -780,6 +782,66 @@ int Engine::nop() {
 int Engine::sync() {
   CKT_HIP(hipSetDevice(device_));
   CKT_HIP(hipStreamSynchronize(stream_));
+  // Every copy out of the staging chunks has run.
+  staging_chunk_ = 0;
+  staging_off_ = 0;
+  return CKT_OK;
+}
+
+// Copies `bytes` into pinned staging memory (new chunks of at least 64 MiB
+// as needed) and returns its address, valid until the next sync().
+int Engine::stage(const void* src, size_t bytes, void** out) {
+  while (staging_chunk_ < staging_.size() &&
+         staging_off_ + bytes > staging_[staging_chunk_].cap) {
+    staging_chunk_++;
+    staging_off_ = 0;
+  }
+  if (staging_chunk_ == staging_.size()) {
+    PinnedChunk c{nullptr, std::max<size_t>(bytes, 64ull << 20)};
+    CKT_HIP(hipHostMalloc(&c.p, c.cap, hipHostMallocDefault));
+    staging_.push_back(c);
+    staging_off_ = 0;
+  }
+  uint8_t* dst = (uint8_t*)staging_[staging_chunk_].p + staging_off_;
+  std::memcpy(dst, src, bytes);
+  staging_off_ = align_up(staging_off_ + bytes, 256);
+  *out = dst;
+  return CKT_OK;
+}
+
+int Engine::add_planned(const DevgenPlan& plan, int32_t loops, uint32_t* id) {
+  CKT_HIP(hipSetDevice(device_));
+  const SlabLayout lay = SlabLayout::make(plan.N, plan.W, plan.P);
+  if (budget_ && in_use_ + lay.total > budget_)
+    return fail(CKT_ERR_BUDGET, "circuit needs " + std::to_string(lay.total) +
+                                    " bytes; budget has " + std::to_string(budget_ - in_use_) + " left");
+  if (plan.wire_state.size() != plan.W) return fail(CKT_ERR_INVALID_ARG, "incomplete generator plan");
+  const size_t need = devgen_scratch_bytes(plan.N, plan.W);
+  if (need == 0) return fail(CKT_ERR_HIP, "device generator scratch size query failed");
+  if (need > devgen_scratch_cap_) {
+    // Builds already enqueued use the old scratch.
+    CKT_HIP(hipStreamSynchronize(stream_));
+    if (devgen_scratch_) CKT_HIP(hipFree(devgen_scratch_));
+    devgen_scratch_ = nullptr;
+    devgen_scratch_cap_ = 0;
+    CKT_HIP(hipMalloc(&devgen_scratch_, need));
+    devgen_scratch_cap_ = need;
+  }
+  void* staged = nullptr;
+  int st;
+  if (plan.W && (st = stage(plan.wire_state.data(), (size_t)plan.W * sizeof(uint64_t), &staged)))
+    return st;
+  void* slab = nullptr;
+  hipError_t e = hipMalloc(&slab, lay.total);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(circuit)");
+  std::vector<ckt_piece_t> pieces;
+  if ((st = devgen_enqueue(plan, (const uint64_t*)staged, lay, slab, devgen_scratch_, stream_,
+                           pieces))) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(slab);
+    return st;
+  }
+  attach(slab, lay, plan.N, plan.W, plan.P, pieces, loops, id);
   return CKT_OK;
 }
 

@@ -51,6 +51,22 @@ int image_from_gen(SlabImage& img, const ckt_gen_params_t* params);
 int devgen_build(const ckt_gen_params_t* params, const SlabLayout& L, void* slab, hipStream_t s,
                  std::vector<ckt_piece_t>& pieces);
 
+// The host part of the device generator, done ahead of time (thread-safe):
+// every wire's drand48 state at its first draw.
+struct DevgenPlan {
+  ckt_gen_params_t params{};
+  uint32_t N = 0, W = 0, P = 0;
+  std::vector<uint64_t> wire_state;
+};
+int devgen_plan(const ckt_gen_params_t* params, DevgenPlan& plan);
+// Device scratch a planned build of N nodes and W wires needs.
+size_t devgen_scratch_bytes(uint32_t N, uint32_t W);
+// Enqueues the planned build on `s` without waiting: `staged_state` (pinned
+// host memory holding plan.wire_state) and `scratch` (devgen_scratch_bytes)
+// must stay untouched until `s` has run the enqueued work.  Fills `pieces`.
+int devgen_enqueue(const DevgenPlan& plan, const uint64_t* staged_state, const SlabLayout& L,
+                   void* slab, void* scratch, hipStream_t s, std::vector<ckt_piece_t>& pieces);
+
 struct Circuit {
   uint32_t id = 0;
   uint32_t num_nodes = 0, num_wires = 0, num_pieces = 0;
@@ -94,6 +110,11 @@ class Engine {
   int add(SlabImage& img, int32_t loops, uint32_t* id);
   // Generates the circuit on the device (no host image, no upload).
   int add_generated_on_device(const ckt_gen_params_t* params, int32_t loops, uint32_t* id);
+  // Same circuit from a plan made earlier: the generation is only enqueued
+  // on the engine stream (after whatever runs there), nothing waits.  The
+  // plan's start states are copied into pinned staging memory that sync()
+  // recycles, so the plan may be dropped on return.
+  int add_planned(const DevgenPlan& plan, int32_t loops, uint32_t* id);
   int remove(uint32_t id);
   int remove_many(const uint32_t* ids, size_t n);
   Circuit* find(uint32_t id);
@@ -115,6 +136,7 @@ class Engine {
   int upv(const std::vector<PieceRef>& refs);
   int sanity(const std::vector<PieceRef>& refs, uint64_t* bad);
   int nop();
+  // Waits for the engine stream; recycles the staging memory of planned adds.
   int sync();
 
   // One loop (cnc, dsc, upv) of `refs` captured in a graph and replayed
@@ -148,6 +170,7 @@ class Engine {
   int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
   int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
   void drop_caches();
+  int stage(const void* src, size_t bytes, void** out);
   int timed_begin(int task, hipEvent_t* a);
   int timed_end(int task, hipEvent_t a);
   int csr_nodes_ok(const std::vector<PieceRef>& refs) const;
@@ -177,6 +200,17 @@ class Engine {
   bool timing_ = false;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_;
+  // Planned device generation: one device scratch (grow-only; the builds on
+  // the stream run one after another) and pinned staging chunks for the
+  // start states, recycled at sync().
+  void* devgen_scratch_ = nullptr;
+  size_t devgen_scratch_cap_ = 0;
+  struct PinnedChunk {
+    void* p;
+    size_t cap;
+  };
+  std::vector<PinnedChunk> staging_;
+  size_t staging_chunk_ = 0, staging_off_ = 0;
   double acc_ms_[CKT_NUM_TASK_IDS] = {0};
   uint64_t acc_n_[CKT_NUM_TASK_IDS] = {0};
 };

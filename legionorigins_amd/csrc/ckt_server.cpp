@@ -5,9 +5,12 @@
 //   arrival   -> FIFO queue
 //   admission -> when the HBM budget, the resident-circuit cap and the
 //                per-iteration wire cap allow, the request's device footprint
-//                is reserved and a host generator thread builds its slab image
-//   upload    -> the image is copied to HBM while the GPU runs the current
-//                iteration; the circuit joins the next iteration
+//                is reserved and a host thread walks its generator chain
+//                (each wire's first drand48 state, 8 bytes per wire)
+//   generate  -> the device generator is enqueued behind the running
+//                iteration (the GPU draws the circuit in HBM; nothing is
+//                uploaded but the start states); the circuit joins the next
+//                iteration
 //   iteration -> one loop (calc_new_currents, distribute_charge,
 //                update_voltages) of every resident circuit with loops left,
 //                each task one launch over all of them
@@ -50,12 +53,12 @@ double now_s() {
 
 struct Ready {
   uint32_t index;
-  std::unique_ptr<SlabImage> img;
+  std::unique_ptr<DevgenPlan> plan;
   int status;
   std::string error;
 };
 
-// Host threads that turn admitted requests into slab images.
+// Host threads that walk admitted requests' generator chains.
 class GenPool {
  public:
   GenPool(int threads, const ckt_request_t* reqs) : reqs_(reqs) {
@@ -98,11 +101,11 @@ class GenPool {
       }
       Ready r;
       r.index = idx;
-      r.img.reset(new SlabImage());
-      r.status = image_from_gen(*r.img, &reqs_[idx].gen);
+      r.plan.reset(new DevgenPlan());
+      r.status = devgen_plan(&reqs_[idx].gen, *r.plan);
       if (r.status) {
         r.error = ckt_last_error();
-        r.img.reset();
+        r.plan.reset();
       }
       {
         std::lock_guard<std::mutex> g(mu_);
@@ -299,11 +302,12 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
       if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) break;
       iterations++;
     }
-    // Overlap: upload circuits whose images are ready while the GPU works.
+    // Overlap: enqueue the generation of planned circuits behind the
+    // running iteration.
     ready.clear();
     double wait = 0;
     if (refs.empty()) {
-      // Idle GPU: sleep until the next arrival or a generated image.
+      // Idle GPU: sleep until the next arrival or a finished plan.
       const double next = next_arrival < order.size() ? reqs[order[next_arrival]].arrival_s : now + 0.05;
       wait = std::max(0.0, std::min(0.05, next - clock()));
       if (admitted == 0 && next_arrival >= order.size() && queue.empty()) wait = 0;
@@ -318,8 +322,8 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
         continue;
       }
       uint32_t id = 0;
-      int as = E.add(*r.img, reqs[r.index].num_loops, &id);
-      r.img.reset();
+      int as = E.add_planned(*r.plan, reqs[r.index].num_loops, &id);
+      r.plan.reset();
       if (as) {
         reserved -= need[r.index];
         admitted_wires -= wires_of[r.index];
