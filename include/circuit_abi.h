@@ -112,7 +112,7 @@ enum {
  *           loop and fused multiply-adds; voltages materialised every step.
  *   FAST  : RECIP with the internal-voltage update substituted into the next
  *           step's current update (a tridiagonal update of the 10 currents per
- *           step, 28 FMAs).  Every step's currents are computed; internal
+ *           step: 20 FMAs and 8 adds).  Every step's currents are computed; internal
  *           voltages are formed from the final currents.
  * RECIP and FAST are not bit-exact; their error is checked against 1e-5. */
 enum { CKT_POLICY_EXACT = 0, CKT_POLICY_RECIP = 1, CKT_POLICY_FAST = 2 };
@@ -370,7 +370,8 @@ typedef struct ckt_server_config {
   int32_t max_active;          /* cap on resident circuits (0 = none) */
   int32_t policy;
   int32_t reduce_mode;
-  int32_t gen_threads;         /* host threads generating admitted circuits */
+  int32_t gen_threads;         /* host threads walking admitted circuits' generator chains (the
+                                  circuits themselves are generated on the device) */
 } ckt_server_config_t;
 
 typedef struct ckt_request {
