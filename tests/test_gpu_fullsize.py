@@ -9,12 +9,15 @@ through size-independent properties.
     followed by the update (checks distribute_charge + update_voltages at
     full size, any reduction order);
   * CSR and ATOMIC reductions agree to 1e-5, and the reference profile gives
-    the oracle's NaN pattern on the sample.
+    the oracle's NaN pattern on the sample;
+  * the benchmark's configuration (FAST, fused ATOMIC) within 1e-5 of the
+    oracle on a sample of wires and on every node voltage.
 """
 import numpy as np
 import pytest
 
-from helpers import TOL, assert_bit_identical, assert_same_class, max_rel_err
+from helpers import (TOL, assert_bit_identical, assert_same_class, max_rel_err,
+                     max_wire_rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,3 +70,33 @@ def test_c2_atomic_and_csr_agree(C):
             ctx.run_loops(2, policy=C.POLICY_FAST, mode=mode)
             out[mode] = ctx.store()[0]
     assert max_rel_err(out[C.REDUCE_ATOMIC]["voltage"], out[C.REDUCE_CSR]["voltage"]) <= TOL
+
+
+def test_c2_fast_sampled_wires_within_tolerance(C, O):
+    # The benchmark's configuration (FAST, fused ATOMIC reduction) at full
+    # size, stable profile: one loop, a random sample of wires against the
+    # oracle's reference-order relaxation (per wire, relative to the wire's
+    # largest segment value), and every node voltage against the float64
+    # scatter-add of the GPU's own currents.
+    P, npp, wpp, pct, seed = C2
+    ref = O.generate(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
+    with C.Context(0) as ctx:
+        ctx.generate_device(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE))
+        ctx.run_loops(1, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC)
+        nodes, wires = ctx.store()
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(len(ref.wires), 1500, replace=False))
+    for k in sample:
+        O.lib().orc_calc_new_currents_f32(0, O.STEPS, O._p(ref.wires), O._p(ref.nodes),
+                                          O._p(ref.nodes), O._p(ref.nodes), int(k), 1)
+    assert np.isfinite(wires["current"][sample]).all()
+    assert max_wire_rel_err(wires["current"][sample], ref.wires["current"][sample]) <= TOL
+    assert max_wire_rel_err(wires["voltage"][sample], ref.wires["voltage"][sample]) <= TOL
+    dt = np.float32(1e-6)
+    q = np.zeros(len(nodes))
+    np.add.at(q, wires["in_ptr"], (-dt * wires["current"][:, 0]).astype(np.float64))
+    np.add.at(q, wires["out_ptr"], (dt * wires["current"][:, 9]).astype(np.float64))
+    want = (ref.nodes["voltage"].astype(np.float64) + q / ref.nodes["capacitance"]) * \
+        (1.0 - ref.nodes["leakage"].astype(np.float64))
+    assert np.isfinite(nodes["voltage"]).all()
+    assert max_rel_err(nodes["voltage"], want) <= TOL
