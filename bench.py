@@ -59,6 +59,8 @@ def parse():
                     help="served: trace length of the batched-vs-sequential comparison")
     ap.add_argument("--upv-sweep", action="store_true",
                     help="update_voltages HBM bandwidth at growing node counts")
+    ap.add_argument("--dsc-sweep", action="store_true",
+                    help="distribute_charge bandwidth per reduction mode at growing sizes")
     ap.add_argument("--gen", action="store_true",
                     help="circuit generation: host generator + upload vs the device generator")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -282,6 +284,45 @@ def run_upv_sweep(C, args):
                       "algorithmic_bytes_per_node": 24, "rows": rows}))
 
 
+def run_dsc_sweep(C, args):
+    """distribute_charge alone, per reduction mode, on circuits of 64/256/1024
+    pieces of 16384 nodes and 65536 wires (95% in-piece).  Algorithmic bytes:
+    ATOMIC streams 16 B per wire (I[0], I[9], in_ptr, out_ptr) and reads and
+    writes each node's charge once (8 B per node); CSR reads 8 B per
+    contribution (index entry, gathered current; 2 per wire) and 12 B per node
+    (row offset, charge read and written)."""
+    rows = []
+    modes = (("atomic_global", C.REDUCE_ATOMIC, 1), ("atomic_lds", C.REDUCE_ATOMIC, 2),
+             ("csr", C.REDUCE_CSR, 0), ("kahan64", C.REDUCE_CSR_KAHAN64, 0))
+    for P in (64, 256, 1024):
+        npp, wpp = 16384, 65536
+        N, W = P * npp, P * wpp
+        ctx = C.Context(0)
+        ctx.generate_device(C.GenParams(P, npp, wpp, 95, 12345, C.PROFILE_STABLE))
+        pieces = ctx.pieces()
+        ctx.set_timing(True)
+        for name, mode, impl in modes:
+            ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, impl)
+            ctx.distribute_charge(pieces, mode)
+            ctx.synchronize()
+            ctx.reset_timing()
+            reps = 20
+            for _ in range(reps):
+                ctx.distribute_charge(pieces, mode)
+            ms, n = ctx.kernel_time(C.DISTRIBUTE_CHARGE)
+            avg = ms / n
+            nbytes = 16 * W + 8 * N if mode == C.REDUCE_ATOMIC else 16 * W + 12 * N
+            gbs = nbytes / (avg * 1e-3) / 1e9
+            rows.append({"wires": W, "nodes": N, "mode": name, "avg_launch_ms": avg,
+                         "algorithmic_bytes": nbytes, "achieved_GBs": gbs,
+                         "frac_of_8TBs": gbs / PEAK_HBM_GBS})
+            log(f"[dsc] {rows[-1]}")
+        ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, 0)
+        ctx.close()
+    print(json.dumps({"metric": "distribute_charge bandwidth per reduction mode", "unit": "GB/s",
+                      "rows": rows}))
+
+
 def run_gen(C, args):
     rows = []
     for P, npp, wpp in ((64, 16384, 65536), (1024, 16384, 65536)):
@@ -333,6 +374,8 @@ def main():
         return run_upv_sweep(C, args)
     if args.gen:
         return run_gen(C, args)
+    if args.dsc_sweep:
+        return run_dsc_sweep(C, args)
 
     policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
     mode = {"atomic": C.REDUCE_ATOMIC, "csr": C.REDUCE_CSR,
