@@ -273,10 +273,22 @@ __global__ __launch_bounds__(1024) void dsc_lds_kernel(const DevCircuit* __restr
 // One lane per node: the node's contributions in (wire, in-before-out) order,
 // added one at a time to its charge -- the order in which the reference's
 // CPU tasks apply them when run one piece after another.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  xcd_tile gives the workgroups of
+// one XCD a contiguous range of the n tiles, so neighbouring node tiles --
+// whose wires, and so whose gathered currents, lie in the same piece -- share
+// an L2 instead of being spread over all eight.  A bijection on [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds;
+  const uint32_t x = b % kXcds, k = b / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template <bool KAHAN>
 __global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restrict__ circuits,
                                                       const Tile* __restrict__ tiles) {
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (threadIdx.x >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
   const uint32_t n = t.first + threadIdx.x;
