@@ -678,7 +678,36 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
   return timed_end(CKT_CALC_NEW_CURRENTS, a);
 }
 
+// Every tile list one loop's launches use, built before the first launch.
+// A synchronous upload issued while a kernel runs waits for CUs the kernel
+// holds (for the whole of a calc_new_currents launch), and the host with it.
+int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
+  int st;
+  if (mode < CKT_REDUCE_ATOMIC || mode > CKT_REDUCE_CSR_KAHAN64)
+    return fail(CKT_ERR_INVALID_ARG, "distribute_charge: unknown reduction mode");
+  // room for every entry made here, so none is evicted before its launch
+  if (tile_cache_.size() + 8 > 64) drop_caches();
+  if ((st = ensure_table())) return st;
+  DevBuf t;
+  if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
+  if (mode == CKT_REDUCE_ATOMIC && !fuse_dsc_) {
+    if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
+    if ((st = tiles_for(TK_PIECES, refs, 0, &t))) return st;
+  }
+  if (mode != CKT_REDUCE_ATOMIC) {
+    if ((st = csr_nodes_ok(refs))) return st;
+    if ((st = tiles_for(TK_CSR, refs, 256, &t))) return st;
+  }
+  return tiles_for(TK_NODES, refs, kUpvTile, &t);
+}
+
 int Engine::loop_once(const std::vector<PieceRef>& refs, int policy, int mode) {
+  int st;
+  if ((st = prefetch_tiles(refs, mode))) return st;
+  return launch_loop(refs, policy, mode);
+}
+
+int Engine::launch_loop(const std::vector<PieceRef>& refs, int policy, int mode) {
   int st;
   if (fuse_dsc_ && mode == CKT_REDUCE_ATOMIC) {
     if ((st = cnc(refs, policy, true))) return st;
@@ -855,23 +884,13 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
   auto it = graphs_.find(key);
   hipGraphExec_t exec = nullptr;
   if (it == graphs_.end()) {
-    // Every buffer the captured launches use is created before capture, in a
-    // cache with room for all of them (a full cache would evict entries made
-    // here and force an allocation during capture).
-    if (tile_cache_.size() + 8 > 64) drop_caches();
-    DevBuf t;
-    if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
-    if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
-    if ((st = tiles_for(TK_PIECES, refs, 0, &t))) return st;
-    if ((st = tiles_for(TK_NODES, refs, kUpvTile, &t))) return st;
-    if (mode != CKT_REDUCE_ATOMIC) {
-      if ((st = csr_nodes_ok(refs))) return st;
-      if ((st = tiles_for(TK_CSR, refs, 256, &t))) return st;
-    }
+    // Every buffer the captured launches use is created before capture (an
+    // allocation during capture is not allowed).
+    if ((st = prefetch_tiles(refs, mode))) return st;
     const bool timing = timing_;
     timing_ = false;
     CKT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    st = loop_once(refs, policy, mode);
+    st = launch_loop(refs, policy, mode);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(stream_, &graph);
     timing_ = timing;

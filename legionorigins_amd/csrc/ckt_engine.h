@@ -169,6 +169,9 @@ class Engine {
   int ensure_table();
   int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
   int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
+  int prefetch_tiles(const std::vector<PieceRef>& refs, int mode);
+  // loop_once's launches; every tile list must already be cached.
+  int launch_loop(const std::vector<PieceRef>& refs, int policy, int mode);
   void drop_caches();
   int stage(const void* src, size_t bytes, void** out);
   int timed_begin(int task, hipEvent_t* a);
