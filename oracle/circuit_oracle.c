@@ -1,7 +1,7 @@
 /*
  * circuit_oracle.c -- CPU restatement of the Legion circuit hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker; see circuit_oracle.h).  Built with
+ * TEST INFRASTRUCTURE ONLY (the oracle; see circuit_oracle.h).  Built with
  * gcc -O3 -ffp-contract=off and no fast-math, so every float operation is a
  * single IEEE-754 binary32 operation as in the reference build
  * (runtime/runtime.mk:38; x86-64 SSE arithmetic has no excess precision).

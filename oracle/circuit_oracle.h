@@ -1,8 +1,9 @@
 /*
  * circuit_oracle.h -- CPU restatement of the Legion circuit hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
- * entry point and the CPU-baseline leg of the benchmark load it.  The product
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the GPU results are
+ * checked against: tests, the smoke entry point and the CPU-baseline leg of
+ * the benchmark load it.  The product
  * (legionorigins_amd) never links, loads or calls it.
  *
  * Each function cites the reference lines it restates (paths relative to the

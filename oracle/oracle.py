@@ -1,8 +1,9 @@
 """ctypes binding of the CPU oracle (``libcircuit_oracle.so``).
 
-TEST INFRASTRUCTURE ONLY: this is the checker.  The record layouts below are
-written out independently of the product package (``apps/circuit/circuit.h``
-lines 35-55 give them) so that a layout mistake in one cannot hide in the other.
+TEST INFRASTRUCTURE ONLY: the oracle that GPU results are checked against.  The
+record layouts below are written out independently of the product package
+(``apps/circuit/circuit.h`` lines 35-55 give them) so that a layout mistake in
+one cannot hide in the other.
 """
 from __future__ import annotations
 

@@ -1,5 +1,5 @@
 """bench.py's command-line contract, checked without a GPU: more than one GPU
-is refused with a message (this build runs on one GPU), and the options the
+is refused with a message (the library runs on one GPU), and the options the
 measurement runs use are accepted."""
 import json
 import os
