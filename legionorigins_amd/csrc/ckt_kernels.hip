@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Wire invariant checker (generalises sanity_check_positions,
+// Wire invariant validator (generalises sanity_check_positions,
 // circuit_gpu.cu:151-184): location codes valid, in-node inside the piece,
 // out-node inside the circuit, and location consistent with the locator and
 // the piece's node range.
