@@ -100,3 +100,31 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O):
         (1.0 - ref.nodes["leakage"].astype(np.float64))
     assert np.isfinite(nodes["voltage"]).all()
     assert max_rel_err(nodes["voltage"], want) <= TOL
+
+
+def test_c2_ten_loops_against_exact_and_fp64(C, O):
+    # C2 for its stated 10 loops, stable profile.  FAST + fused ATOMIC (the
+    # benchmark's configuration) against EXACT + CSR -- bit-identical to the
+    # CPU restatement at C1/C3 and on the sampled C2 wires above, so it stands
+    # in for the CPU run, which would take ~15 core-hours here -- on every node
+    # voltage; both against the FP64 oracle (the relaxation's FP64 limit),
+    # printed with -rP.
+    P, npp, wpp, pct, seed = C2
+    gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
+    out = {}
+    for name, policy, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
+                               ("fast+atomic", C.POLICY_FAST, C.REDUCE_ATOMIC)):
+        with C.Context(0) as ctx:
+            ctx.generate_device(gp)
+            ctx.run_loops(10, policy=policy, mode=mode, use_graph=True)
+            out[name] = ctx.store()[0]["voltage"]
+        assert np.isfinite(out[name]).all()
+    truth, _ = O.run_f64_limit(O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE), 10)
+    tv = truth["voltage"]
+    errs = {k: max_rel_err(v, tv) for k, v in out.items()}
+    diff = max_rel_err(out["fast+atomic"], out["exact+csr"])
+    print(f"\nC2, 10 loops, stable profile: max relative node-voltage error vs FP64 "
+          f"exact+csr {errs['exact+csr']:.3e}, fast+atomic {errs['fast+atomic']:.3e}; "
+          f"fast+atomic vs exact+csr {diff:.3e}")
+    assert diff <= TOL
+    assert max(errs.values()) <= 1e-3
