@@ -105,10 +105,10 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O):
 def test_c2_ten_loops_against_exact_and_fp64(C, O):
     # C2 for its stated 10 loops, stable profile.  FAST + fused ATOMIC (the
     # benchmark's configuration) against EXACT + CSR -- bit-identical to the
-    # CPU restatement at C1/C3 and on the sampled C2 wires above, so it stands
-    # in for the CPU run, which would take ~15 core-hours here -- on every node
-    # voltage; both against the FP64 oracle (the relaxation's FP64 limit),
-    # printed with -rP.
+    # CPU restatement at C1/C3, on the sampled C2 wires above and over all of
+    # C2 for 10 loops (tests/c2_exact_vs_oracle.py, ~1.7 core-hours of CPU),
+    # so it stands in for the CPU run -- on every node voltage; both against
+    # the FP64 oracle (the relaxation's FP64 limit), printed with -rP.
     P, npp, wpp, pct, seed = C2
     gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
     out = {}
