@@ -26,12 +26,30 @@ namespace ckt {
 template <int POLICY>
 struct Relax;
 
+// Correctly rounded a / b for a divisor that stays fixed across the step
+// loop.  With rb = RN64(1/b) (one FP64 division per wire), RN32((double)a * rb)
+// equals RN32(a / b) whenever the quotient is a normal float: a quotient of
+// two floats is never exactly halfway between two normal floats and lies at
+// least 2^-49 (relative) from every such midpoint, while the FP64 product is
+// within 2^-52 of a / b, so both round the same way (the double-rounding
+// argument of S. Figueroa, "When is double rounding innocuous?", 1995).  Zero
+// and subnormal quotients take the IEEE division instead.  Infinite, NaN and
+// zero operands give what the IEEE division gives.
+__device__ __forceinline__ float div_fixed(float a, double rb, float b) {
+  float q = (float)((double)a * rb);
+  if (__builtin_expect(fabsf(q) < 0x1p-126f && a != 0.f, 0)) q = a / b;
+  return q;
+}
+
 // Reference order of operations; bit-identical to a strict FP32 CPU run.
+// Every division is correctly rounded (div_fixed).
 template <>
 struct Relax<CKT_POLICY_EXACT> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
     const float dt = CKT_DT_F;
+    constexpr double rdt = 1.0 / (double)CKT_DT_F;
+    const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
     float ni[kSeg], nv[kSeg + 1];
 #pragma unroll
     for (int i = 0; i < kSeg; i++) ni[i] = I[i];
@@ -41,9 +59,11 @@ struct Relax<CKT_POLICY_EXACT> {
     nv[kSeg] = vout;
     for (int j = 0; j < steps; j++) {
 #pragma unroll
-      for (int i = 0; i < kSeg; i++) ni[i] = ((nv[i + 1] - nv[i]) - (L * (ni[i] - I[i])) / dt) / R;
+      for (int i = 0; i < kSeg; i++)
+        ni[i] = div_fixed((nv[i + 1] - nv[i]) - div_fixed(L * (ni[i] - I[i]), rdt, dt), rR, R);
 #pragma unroll
-      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i] + (dt * (ni[i] - ni[i + 1])) / C;
+      for (int i = 0; i < kSeg - 1; i++)
+        nv[i + 1] = V[i] + div_fixed(dt * (ni[i] - ni[i + 1]), rC, C);
     }
 #pragma unroll
     for (int i = 0; i < kSeg; i++) I[i] = ni[i];
