@@ -106,8 +106,9 @@ enum {
 };
 
 /* Arithmetic policy of calc_new_currents.
- *   EXACT : the reference kernel's operation order with IEEE-754 division
- *           (circuit_gpu.cu:114-129); bit-identical to a strict FP32 CPU run.
+ *   EXACT : the reference kernel's operation order with correctly rounded
+ *           division (circuit_gpu.cu:114-129); bit-identical to a strict FP32
+ *           CPU run.
  *   RECIP : per-wire reciprocals (1/R, L/dt, dt/C) hoisted out of the step
  *           loop and fused multiply-adds; voltages materialised every step.
  *   FAST  : RECIP with the internal-voltage update substituted into the next
