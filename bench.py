@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector/matrix peak (public spec)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (public spec)
 FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
+EXECUTED_FLOPS_FAST = 48   # FAST step: 20 FMA (2 flops) + 8 add
 
 
 def log(msg):
@@ -427,6 +428,14 @@ def main():
                      "flops_per_launch": FLOPS_PER_WIRE_STEP * C.STEPS * W,
                      "avg_launch_ms": cnc_ms, "launches": cnc_n},
     }
+    if args.policy == "fast":
+        # FP32 work the FAST step executes: 20 FMA + 8 add per wire-step (the
+        # kernel's instruction mix; the VALU counters agree), against the
+        # algorithmic 96 of the reference formula above.
+        executed = EXECUTED_FLOPS_FAST * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
+        out["roofline"]["executed"] = {
+            "flops_per_wire_step": EXECUTED_FLOPS_FAST, "tflops": executed,
+            "frac": executed / PEAK_FP32_TFLOPS}
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
