@@ -62,6 +62,9 @@ def parse():
                     help="update_voltages HBM bandwidth at growing node counts")
     ap.add_argument("--dsc-sweep", action="store_true",
                     help="distribute_charge bandwidth per reduction mode at growing sizes")
+    ap.add_argument("--accessor", action="store_true",
+                    help="the drop-in accessor form at C2: per-piece synchronous calls on "
+                         "reference-layout instances")
     ap.add_argument("--gen", action="store_true",
                     help="circuit generation: host generator + upload vs the device generator")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -324,6 +327,66 @@ def run_dsc_sweep(C, args):
                       "rows": rows}))
 
 
+def run_accessor(C, args):
+    """The three tasks through the accessor-form entry points -- what the
+    reference's GPU task wrappers call (circuit.cc:475-550) -- on C2 in the
+    reference layouts (128-byte wires, 16-byte nodes) in device memory: one
+    call per piece per task, each waiting for its kernel, as the reference
+    runtime runs GPU point tasks.  On one GPU the private, shared and ghost
+    node instances are one array; shared and ghost charges go to fold buffers
+    that are applied once per loop (the runtime's reduction-instance apply).
+    Beside it, the resident context in its per-piece mode and in its normal
+    one-launch-per-task mode, on the same circuit."""
+    import numpy as np
+    wl = dict(WORKLOADS["c2"])
+    wl.pop("loops")
+    policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
+    params = C.GenParams(**wl, profile=C.PROFILE_REFERENCE)
+    host = C.generate(params)
+    N, W = len(host.nodes), len(host.wires)
+    dw = C.DeviceArray(host.wires)
+    nodes = C.DeviceArray(host.nodes)
+    loc = C.DeviceArray(host.locator)
+    ofold = C.DeviceArray(np.zeros(N, np.float32))
+    gfold = C.DeviceArray(np.zeros(N, np.float32))
+    pieces = list(host.pieces)
+
+    def loop():
+        for p in pieces:
+            C.calc_new_currents_gpu(p, dw, nodes, nodes, nodes, flag=policy)
+        C.fold_init_gpu(ofold, 0, N)
+        C.fold_init_gpu(gfold, 0, N)
+        for p in pieces:
+            C.distribute_charge_gpu(p, dw, nodes, ofold, gfold)
+        C.fold_apply_gpu(nodes, ofold, 0, N)
+        C.fold_apply_gpu(nodes, gfold, 0, N)
+        for p in pieces:
+            C.update_voltages_gpu(p, nodes, nodes, loc)
+
+    loop()
+    loops = 3
+    t0 = time.perf_counter()
+    for _ in range(loops):
+        loop()
+    acc = W * loops / (time.perf_counter() - t0)
+    log(f"[accessor] {acc:.4g} wire-updates/s")
+    rows = {"accessor_form_per_piece_sync": acc}
+    ctx = C.Context(0)
+    ctx.generate(params)
+    ctx.set_option(C.OPT_FUSE_DSC, 0)
+    ctx.run_loops(1, policy=policy, mode=C.REDUCE_ATOMIC, per_piece=True)
+    rows["context_per_piece_sync"] = ctx.run_loops(
+        loops, policy=policy, mode=C.REDUCE_ATOMIC, per_piece=True)["wire_updates_per_s"]
+    ctx.set_option(C.OPT_FUSE_DSC, 1)
+    ctx.run_loops(1, policy=policy, mode=C.REDUCE_ATOMIC)
+    rows["context_index_space"] = ctx.run_loops(
+        loops, policy=policy, mode=C.REDUCE_ATOMIC)["wire_updates_per_s"]
+    ctx.close()
+    log(f"[accessor] {rows}")
+    print(json.dumps({"metric": "wire-updates/s at C2 by calling form", "unit": "wire-updates/s",
+                      "policy": args.policy, "loops": loops, "rows": rows}))
+
+
 def run_gen(C, args):
     rows = []
     for P, npp, wpp in ((64, 16384, 65536), (1024, 16384, 65536)):
@@ -377,6 +440,8 @@ def main():
         return run_gen(C, args)
     if args.dsc_sweep:
         return run_dsc_sweep(C, args)
+    if args.accessor:
+        return run_accessor(C, args)
 
     policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
     mode = {"atomic": C.REDUCE_ATOMIC, "csr": C.REDUCE_CSR,

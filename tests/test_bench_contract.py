@@ -25,5 +25,5 @@ def test_help_lists_the_measurement_modes():
     r = run("--help")
     assert r.returncode == 0
     for flag in ("--steps", "--warmup", "--gpus", "--batched", "--served", "--upv-sweep",
-                 "--dsc-sweep", "--gen", "--profile", "--copy-calibration"):
+                 "--dsc-sweep", "--gen", "--accessor", "--profile", "--copy-calibration"):
         assert flag in r.stdout, flag
