@@ -229,9 +229,11 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   drop_caches();
   for (Circuit* c : slots_) {
-    (void)hipFree(c->slab);
+    if (c->stream_ordered) (void)hipFreeAsync(c->slab, stream_);
+    else (void)hipFree(c->slab);
     delete c;
   }
+  if (stream_) (void)hipStreamSynchronize(stream_);
   slots_.clear();
   if (d_table_) (void)hipFree(d_table_);
   if (d_counters_) (void)hipFree(d_counters_);
@@ -257,6 +259,14 @@ int Engine::init(int device, uint64_t budget_bytes) {
   lds_limit_ = lds;
   budget_ = budget_bytes ? budget_bytes : (uint64_t)(0.9 * (double)hbm);
   CKT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // Planned circuits come from the device's default memory pool; keep what
+  // they release for reuse instead of returning it at each synchronisation.
+  // Without pool support, planned circuits use plain allocations instead.
+  hipMemPool_t pool = nullptr;
+  uint64_t keep = UINT64_MAX;
+  pool_ok_ = hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess &&
+             hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+  (void)hipGetLastError();
   CKT_HIP(hipMalloc((void**)&d_counters_, 4 * sizeof(uint32_t)));
   CKT_HIP(hipMalloc((void**)&d_bad_, sizeof(unsigned long long)));
   CKT_HIP(launch_fill_u32(d_counters_, 0u, 4, stream_));
@@ -304,8 +314,10 @@ int Engine::add_generated_on_device(const ckt_gen_params_t* params, int32_t loop
 }
 
 void Engine::attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uint32_t P,
-                    const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id) {
+                    const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id,
+                    bool stream_ordered) {
   Circuit* c = new Circuit();
+  c->stream_ordered = stream_ordered;
   c->id = next_id_++;
   c->num_nodes = N;
   c->num_wires = W;
@@ -375,22 +387,32 @@ void Engine::attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uin
 
 int Engine::remove(uint32_t id) { return remove_many(&id, 1); }
 
-// One stream synchronisation for the whole set.  Cached tile lists and graphs
-// are keyed by the engine generation, which changes here, so entries naming
-// the removed circuits can never be looked up again; they stay until the
-// cache's size bound evicts them (drop_caches waits for the stream first).
+// A slab from the stream's memory pool is released stream-ordered, after
+// everything already queued on the stream, so removing only such circuits
+// needs no synchronisation; any other slab is freed after one stream
+// synchronisation for the whole set.  The device descriptor table is only
+// rewritten by ensure_table, which waits for the stream first.  Cached tile
+// lists and graphs are keyed by the engine generation, which changes here, so
+// entries naming the removed circuits can never be looked up again; they stay
+// until the cache's size bound evicts them (drop_caches waits for the stream
+// first).
 int Engine::remove_many(const uint32_t* ids, size_t n) {
-  for (size_t k = 0; k < n; k++)
-    if (!by_id_.count(ids[k]))
+  bool sync = false;
+  for (size_t k = 0; k < n; k++) {
+    auto it = by_id_.find(ids[k]);
+    if (it == by_id_.end())
       return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(ids[k]));
+    if (!it->second->stream_ordered) sync = true;
+  }
   if (n == 0) return CKT_OK;
   CKT_HIP(hipSetDevice(device_));
-  CKT_HIP(hipStreamSynchronize(stream_));
+  if (sync) CKT_HIP(hipStreamSynchronize(stream_));
   for (size_t k = 0; k < n; k++) {
     auto it = by_id_.find(ids[k]);
     if (it == by_id_.end()) continue;  // listed twice
     Circuit* c = it->second;
-    CKT_HIP(hipFree(c->slab));
+    if (c->stream_ordered) CKT_HIP(hipFreeAsync(c->slab, stream_));
+    else CKT_HIP(hipFree(c->slab));
     in_use_ -= c->lay.total;
     slots_.erase(std::find(slots_.begin(), slots_.end(), c));
     by_id_.erase(it);
@@ -860,17 +882,21 @@ int Engine::add_planned(const DevgenPlan& plan, int32_t loops, uint32_t* id) {
   int st;
   if (plan.W && (st = stage(plan.wire_state.data(), (size_t)plan.W * sizeof(uint64_t), &staged)))
     return st;
+  // From the stream's memory pool: allocated and, at removal, released in
+  // stream order, without waiting for the device.
   void* slab = nullptr;
-  hipError_t e = hipMalloc(&slab, lay.total);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(circuit)");
+  hipError_t e = pool_ok_ ? hipMallocAsync(&slab, lay.total, stream_) : hipMalloc(&slab, lay.total);
+  if (e != hipSuccess) return hip_fail(e, "allocating a circuit");
   std::vector<ckt_piece_t> pieces;
   if ((st = devgen_enqueue(plan, (const uint64_t*)staged, lay, slab, devgen_scratch_, stream_,
                            pieces))) {
     (void)hipStreamSynchronize(stream_);
-    (void)hipFree(slab);
+    if (pool_ok_) (void)hipFreeAsync(slab, stream_);
+    else (void)hipFree(slab);
+    (void)hipStreamSynchronize(stream_);
     return st;
   }
-  attach(slab, lay, plan.N, plan.W, plan.P, pieces, loops, id);
+  attach(slab, lay, plan.N, plan.W, plan.P, pieces, loops, id, pool_ok_);
   return CKT_OK;
 }
 

@@ -77,6 +77,7 @@ struct Circuit {
   std::unordered_map<uint64_t, uint32_t> piece_index;  // descriptor -> index
   bool csr_ok = false;   // the pieces partition the wires
   bool lds_ok = false;   // node ranges of the pieces are disjoint
+  bool stream_ordered = false;  // slab from the engine stream's memory pool
   int32_t loops_left = 0;
 };
 
@@ -165,7 +166,8 @@ class Engine {
     hipEvent_t a, b;
   };
   void attach(void* slab, const SlabLayout& L, uint32_t N, uint32_t W, uint32_t P,
-              const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id);
+              const std::vector<ckt_piece_t>& pieces, int32_t loops, uint32_t* id,
+              bool stream_ordered = false);
   int ensure_table();
   int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
   int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
@@ -206,6 +208,7 @@ class Engine {
   // Planned device generation: one device scratch (grow-only; the builds on
   // the stream run one after another) and pinned staging chunks for the
   // start states, recycled at sync().
+  bool pool_ok_ = false;  // the device's default memory pool is usable
   void* devgen_scratch_ = nullptr;
   size_t devgen_scratch_cap_ = 0;
   struct PinnedChunk {
