@@ -514,6 +514,14 @@ int Engine::store_topology(uint32_t id, int32_t* locator, uint32_t* csr_row, uin
   return CKT_OK;
 }
 
+int Engine::copy_voltages_async(uint32_t id, float* dst) {
+  Circuit* c = find(id);
+  if (!c) return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(id));
+  CKT_HIP(hipMemcpyAsync(dst, c->dev.voltage, (size_t)c->num_nodes * 4, hipMemcpyDeviceToHost,
+                         stream_));
+  return CKT_OK;
+}
+
 int Engine::download_voltages(uint32_t id, std::vector<float>& v) {
   Circuit* c = find(id);
   if (!c) return fail(CKT_ERR_NOT_FOUND, "no circuit with id " + std::to_string(id));

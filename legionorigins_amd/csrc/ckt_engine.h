@@ -122,6 +122,9 @@ class Engine {
   int store(uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
   int store_topology(uint32_t id, int32_t* locator, uint32_t* csr_row, uint32_t* csr_ent);
   int download_voltages(uint32_t id, std::vector<float>& v);
+  // Enqueues a copy of the node voltages into `dst` (pinned host memory, at
+  // least num_nodes floats) on the engine stream; valid after sync().
+  int copy_voltages_async(uint32_t id, float* dst);
   size_t count() const { return slots_.size(); }
   uint64_t bytes_in_use() const { return in_use_; }
   uint64_t budget() const { return budget_; }

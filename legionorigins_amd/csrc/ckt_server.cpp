@@ -123,6 +123,29 @@ class GenPool {
   bool stop_ = false;
 };
 
+// Grow-only pinned host buffer (device-to-host copies into it are
+// asynchronous); grown only when no copy into it is pending.
+struct PinnedBuffer {
+  void* p = nullptr;
+  size_t cap = 0;
+  ~PinnedBuffer() {
+    if (p) (void)hipHostFree(p);
+  }
+  int reserve(size_t bytes) {
+    if (bytes <= cap) return CKT_OK;
+    if (p) CKT_HIP(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    CKT_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    cap = bytes;
+    return CKT_OK;
+  }
+  template <class T>
+  T* at(size_t elem_offset) {
+    return reinterpret_cast<T*>(p) + elem_offset;
+  }
+};
+
 double percentile(std::vector<double> v, double p) {
   if (v.empty()) return 0.0;
   std::sort(v.begin(), v.end());
@@ -255,6 +278,9 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
 
   std::vector<Ready> ready;
   std::vector<uint32_t> in_flight;  // requests advanced by the running iteration
+  // (circuit id, offset in `downloads`) of circuits finishing this iteration
+  std::vector<std::pair<uint32_t, size_t>> finishing;
+  PinnedBuffer downloads;
   while (finished < n && !st) {
     const double now = clock();
     // Arrivals.
@@ -301,6 +327,21 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
     if (!refs.empty()) {
       if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) break;
       iterations++;
+      // Circuits that finish with this iteration: their final voltages are
+      // copied to pinned memory behind it, for the checksum after the sync.
+      finishing.clear();
+      size_t total = 0;
+      for (uint32_t ri : in_flight) {
+        const Circuit* c = E.find((uint32_t)circuit_of[ri]);
+        if (c->loops_left == 1) {
+          finishing.emplace_back(c->id, total);
+          total += c->num_nodes;
+        }
+      }
+      if ((st = downloads.reserve(total * sizeof(float)))) break;
+      for (auto& f : finishing)
+        if ((st = E.copy_voltages_async(f.first, downloads.at<float>(f.second)))) break;
+      if (st) break;
     }
     // Overlap: enqueue the generation of planned circuits behind the
     // running iteration.
@@ -352,10 +393,17 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
       if (c->loops_left <= 0) done.push_back(c->id);
     for (uint32_t id : done) {
       const uint32_t ri = request_of_circuit[id];
-      std::vector<float> v;
-      if ((st = E.download_voltages(id, v))) break;
       double sum = 0;
-      for (float x : v) sum += x;
+      auto f = std::find_if(finishing.begin(), finishing.end(),
+                            [id](const std::pair<uint32_t, size_t>& x) { return x.first == id; });
+      if (f != finishing.end()) {
+        const float* v = downloads.at<float>(f->second);
+        for (uint32_t k = 0, n = E.find(id)->num_nodes; k < n; k++) sum += v[k];
+      } else {  // never ran (zero loops)
+        std::vector<float> v;
+        if ((st = E.download_voltages(id, v))) break;
+        for (float x : v) sum += x;
+      }
       results[ri].final_voltage_checksum = sum;
       if (s->keep_state) {
         Circuit* c = E.find(id);
