@@ -14,8 +14,9 @@
 //   iteration -> one loop (calc_new_currents, distribute_charge,
 //                update_voltages) of every resident circuit with loops left,
 //                each task one launch over all of them
-//   retire    -> a circuit whose loops are done is checksummed, optionally
-//                kept, and its slab freed.
+//   retire    -> a circuit whose loops are done is checksummed (its final
+//                voltages are copied behind its last iteration), optionally
+//                kept, and its slab released.
 // Circuits are independent, so a circuit's result does not depend on what
 // else shares its iterations (bit-identical with EXACT + CSR).
 #include <time.h>
@@ -413,7 +414,8 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
       }
     }
     if (st) break;
-    // One removal for the whole set: a single stream synchronisation.
+    // One removal for the whole set; slabs from the stream's memory pool are
+    // released in stream order, without waiting.
     if ((st = E.remove_many(done.data(), done.size()))) break;
     for (uint32_t id : done) {
       const uint32_t ri = request_of_circuit[id];
