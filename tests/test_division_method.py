@@ -1,0 +1,65 @@
+"""The property the EXACT policy's division rests on, checked on the CPU with
+IEEE float32/float64 arithmetic (numpy): for float a and b, with
+rb = RN64(1/b),
+
+    RN32(RN64(a * rb)) == RN32(a / b)   whenever a / b is a normal float.
+
+A quotient of two floats is never exactly halfway between two normal floats
+and stays at least 2^-49 (relative) from every such midpoint, while the FP64
+product is within 2^-52 of a / b (S. Figueroa, "When is double rounding
+innocuous?", 1995).  The kernel sends zero and subnormal quotients to the IEEE
+division; this test checks the normal range, with random operands, with the
+divisors the circuit uses (dt, R in [1, 11], C in [0.001, 0.011]) and with
+dividends placed within two ulps of b*m for random midpoints m.
+"""
+import numpy as np
+
+FLT_MIN = np.float32(2.0 ** -126)
+
+
+def method(a, b):
+    rb = 1.0 / b.astype(np.float64)
+    return (a.astype(np.float64) * rb).astype(np.float32)
+
+
+def check(a, b):
+    with np.errstate(all="ignore"):
+        want = a / b
+        got = method(a, b)
+    normal = np.isfinite(want) & (np.abs(want) >= FLT_MIN)
+    assert normal.mean() > 0.5
+    bad = got[normal].view(np.uint32) != want[normal].view(np.uint32)
+    assert not bad.any(), (a[normal][bad][:3], b[normal][bad][:3])
+
+
+def test_random_operands():
+    rng = np.random.default_rng(0)
+    for _ in range(10):
+        a = rng.integers(0, 2**32, 2_000_000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+        b = rng.integers(0, 2**32, 2_000_000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+        check(a, b)
+
+
+def test_circuit_divisors():
+    rng = np.random.default_rng(1)
+    n = 3_000_000
+    a = (rng.standard_normal(n) * 10.0 ** rng.uniform(-30, 30, n)).astype(np.float32)
+    for b in (np.full(n, 1e-6, np.float32),
+              rng.uniform(1, 11, n).astype(np.float32),
+              rng.uniform(0.001, 0.011, n).astype(np.float32)):
+        check(a, b)
+
+
+def test_dividends_next_to_midpoints():
+    rng = np.random.default_rng(2)
+    n = 2_000_000
+    b = np.ldexp(1.0 + rng.integers(0, 2**23, n) / 2.0**23, rng.integers(-30, 30, n))
+    b = b.astype(np.float32)
+    # midpoint m = (2^24 + k + 1/2) * 2^e: 25 significant bits
+    m = np.ldexp((2.0**24 + rng.integers(0, 2**24, n) + 0.5), rng.integers(-54, 6, n))
+    a0 = (m * b.astype(np.float64)).astype(np.float32)
+    for d in (-2, -1, 0, 1, 2):
+        a = a0.copy()
+        for _ in range(abs(d)):
+            a = np.nextafter(a, np.float32(np.inf) if d > 0 else np.float32(-np.inf))
+        check(a, b)
