@@ -160,6 +160,26 @@ struct Relax<CKT_POLICY_FAST> {
     for (int i = 0; i < kSeg - 1; i++) V[i] = fmaf(b, x[i] - x[i + 1], V[i]);
 #pragma unroll
     for (int i = 0; i < kSeg; i++) I[i] = x[i];
+    // Non-finite states.  In the reference recurrence a current that turns
+    // +-inf is NaN one step later: the voltage difference across its segment
+    // is then +-inf or NaN with the current's sign, and the inductive term
+    // L*(I - I0)/dt is the same infinity.  NaN then spreads one segment per
+    // step through the voltages, so a wire that leaves the finite range ends
+    // with every current and internal voltage NaN (the reference parameters
+    // do this within ~10 of the 10000 steps).  The substitution above has no
+    // voltages in its step, so same-signed infinite currents would stay
+    // infinite; non-finite values never become finite again in it, so the
+    // end state says whether the wire left the finite range.
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) finite &= __builtin_isfinite(x[i]);
+    if (!finite) {
+      const float nan = __builtin_nanf("");
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) I[i] = nan;
+#pragma unroll
+      for (int i = 0; i < kSeg - 1; i++) V[i] = nan;
+    }
   }
 };
 

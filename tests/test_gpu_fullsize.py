@@ -11,7 +11,8 @@ through size-independent properties.
   * CSR and ATOMIC reductions agree to 1e-5, and the reference profile gives
     the oracle's NaN pattern on the sample;
   * the benchmark's configuration (FAST, fused ATOMIC) within 1e-5 of the
-    oracle on a sample of wires and on every node voltage.
+    oracle on a sample of wires and on every node voltage, and with the
+    oracle's non-finite classes under the reference parameters.
 """
 import numpy as np
 import pytest
@@ -100,6 +101,33 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O):
         (1.0 - ref.nodes["leakage"].astype(np.float64))
     assert np.isfinite(nodes["voltage"]).all()
     assert max_rel_err(nodes["voltage"], want) <= TOL
+
+
+def test_c2_fast_reference_profile_classes_after_one_loop(C, O):
+    # The benchmark's configuration and profile (reference parameters, FAST,
+    # fused ATOMIC), one loop: every sampled wire's currents and internal
+    # voltages have the oracle's non-finite classes, and every node voltage
+    # has the class of EXACT + CSR's (bit-identical to the oracle at C2).
+    P, npp, wpp, pct, seed = C2
+    gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_REFERENCE)
+    got = {}
+    for name, policy, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
+                               ("fast+atomic", C.POLICY_FAST, C.REDUCE_ATOMIC)):
+        with C.Context(0) as ctx:
+            ctx.generate_device(gp)
+            ctx.run_loops(1, policy=policy, mode=mode)
+            got[name] = ctx.store()
+    ref = O.generate(P, npp, wpp, pct, seed, O.PROFILE_REFERENCE)
+    rng = np.random.default_rng(2)
+    sample = np.sort(rng.choice(len(ref.wires), 1500, replace=False))
+    for k in sample:
+        O.lib().orc_calc_new_currents_f32(0, O.STEPS, O._p(ref.wires), O._p(ref.nodes),
+                                          O._p(ref.nodes), O._p(ref.nodes), int(k), 1)
+    wires = got["fast+atomic"][1]
+    assert_same_class(wires["current"][sample], ref.wires["current"][sample], "sampled currents")
+    assert_same_class(wires["voltage"][sample], ref.wires["voltage"][sample], "sampled voltages")
+    assert_same_class(got["fast+atomic"][0]["voltage"], got["exact+csr"][0]["voltage"],
+                      "node voltage")
 
 
 def test_c2_ten_loops_against_exact_and_fp64(C, O):
