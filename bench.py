@@ -26,7 +26,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector/matrix peak (public spec)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (public spec)
 FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
-EXECUTED_FLOPS_FAST = 48   # FAST step: 20 FMA (2 flops) + 8 add
+# FP32 flops each policy's step executes (its loop body's instruction mix):
+# FAST 20 FMA + 8 add; SCALED 10 FMA + 8 add, plus 10 multiplies per 4 steps.
+EXECUTED_FLOPS = {"fast": 48, "scaled": 30.5}
 
 
 def log(msg):
@@ -38,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed loops")
     ap.add_argument("--warmup", type=int, default=2, help="untimed loops")
-    ap.add_argument("--policy", default="fast", choices=["exact", "recip", "fast"])
+    ap.add_argument("--policy", default="scaled", choices=["exact", "recip", "fast", "scaled"])
     ap.add_argument("--mode", default="atomic", choices=["atomic", "csr", "kahan64"])
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--profile", default="reference", choices=["reference", "stable"],
@@ -128,8 +130,14 @@ def batched_shapes(C, n=512, lo=2**10, hi=2**18, seed=4, profile=None):
             for i in range(n)]
 
 
+def policy_of(C, name):
+    return {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST,
+            "scaled": C.POLICY_SCALED}[name]
+
+
 def run_batched(C, args):
     loops = 5
+    pol = policy_of(C, args.policy)
     shapes = batched_shapes(C)
     W_total = sum(s.num_pieces * s.wires_per_piece for s in shapes)
     b = C.Batch(0)
@@ -137,10 +145,10 @@ def run_batched(C, args):
     b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, os.cpu_count() or 1))
     log(f"[batched] {len(shapes)} circuits, {W_total} wires, added in "
         f"{time.perf_counter() - t0:.1f} s")
-    best, probe_ms = b.autotune(C.POLICY_FAST, probe_steps=500)
+    best, probe_ms = b.autotune(pol, probe_steps=500)
     log(f"[batched] autotune: block={best.block_threads} persistent={best.persistent} "
         f"waves/SIMD={best.waves_per_simd} ({probe_ms:.2f} ms per 500-step probe)")
-    best.policy, best.reduce_mode, best.use_graph = C.POLICY_FAST, C.REDUCE_ATOMIC, 1
+    best.policy, best.reduce_mode, best.use_graph = pol, C.REDUCE_ATOMIC, 1
     b.set_timing(True)
     rep = b.run(best)
     b.close()
@@ -155,12 +163,12 @@ def run_batched(C, args):
     for i, s in enumerate(shapes):
         ctx.generate(s)
         ctx.set_option(C.OPT_FUSE_DSC, 0)
-        seq_piece += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+        seq_piece += ctx.run_loops(loops, policy=pol, mode=C.REDUCE_ATOMIC,
                                    per_piece=True)["elapsed_s"]
-        seq_plain += ctx.run_loops(loops, policy=C.POLICY_FAST,
+        seq_plain += ctx.run_loops(loops, policy=pol,
                                    mode=C.REDUCE_ATOMIC)["elapsed_s"]
         ctx.set_option(C.OPT_FUSE_DSC, 1)
-        seq_task += ctx.run_loops(loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+        seq_task += ctx.run_loops(loops, policy=pol, mode=C.REDUCE_ATOMIC,
                                   use_graph=True)["elapsed_s"]
         if i % 64 == 63:
             log(f"[batched] sequential: {i + 1} circuits")
@@ -171,7 +179,8 @@ def run_batched(C, args):
            "dtype": "f32", "data": "synthetic (load_circuit generator, seeds 1000+i)",
            "config": {"workload": "512 circuits, wires log-uniform in [2^10, 2^18], nodes = wires/4, "
                                   "pieces = ceil(nodes/4096), 95% in-piece, 10000 inner steps, "
-                                  "5 loops each, FAST policy, ATOMIC reduction, hipGraph per loop",
+                                  f"5 loops each, {args.policy.upper()} policy, ATOMIC reduction, "
+                                  "hipGraph per loop",
                       "total_wires": W_total,
                       "autotuned": {"block_threads": best.block_threads,
                                     "persistent": best.persistent,
@@ -188,13 +197,14 @@ def run_batched(C, args):
 
 def run_served(C, args):
     budget = 200 * 10**9
+    pol = policy_of(C, args.policy)
     trace = C.poisson_trace(args.serve_rate, args.serve_seconds, seed=5,
                             timeout_s=args.serve_timeout)
     offered = sum(r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops for r in trace)
     log(f"[served] {len(trace)} requests over {args.serve_seconds} s, "
         f"{offered:.3g} wire-updates offered ({offered / args.serve_seconds:.3g}/s)")
     s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
-                 policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC, gen_threads=12)
+                 policy=pol, mode=C.REDUCE_ATOMIC, gen_threads=12)
     res, rep = s.run_trace(trace)
     s.close()
     log(f"[served] {rep}")
@@ -203,7 +213,7 @@ def run_served(C, args):
     cmp = {}
     for name, max_active in (("continuous", 0), ("sequential", 1)):
         s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
-                     max_active=max_active, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+                     max_active=max_active, policy=pol, mode=C.REDUCE_ATOMIC,
                      gen_threads=12)
         _, r2 = s.run_trace(short)
         s.close()
@@ -224,7 +234,7 @@ def run_served(C, args):
         free_at, lat, wu = 0.0, [], 0.0
         for r in short:
             ctx.generate(r.gen)
-            svc = ctx.run_loops(r.num_loops, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC,
+            svc = ctx.run_loops(r.num_loops, policy=pol, mode=C.REDUCE_ATOMIC,
                                 per_piece=per_piece)["elapsed_s"]
             start = max(r.arrival_s, free_at)
             free_at = start + svc
@@ -244,7 +254,7 @@ def run_served(C, args):
            "config": {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s; "
                                   "wires log-uniform in [2^12, 2^20], nodes = wires/4, "
                                   "pieces = ceil(nodes/4096), 95% in-piece, loops U[1,20], "
-                                  "FAST policy, ATOMIC reduction",
+                                  f"{args.policy.upper()} policy, ATOMIC reduction",
                       "hbm_budget_bytes": budget, "max_active_wires": args.serve_cap_wires},
            "latency_p50_s": rep["latency_p50_s"], "latency_p99_s": rep["latency_p99_s"],
            "latency_mean_s": rep["latency_mean_s"], "latency_max_s": rep["latency_max_s"],
@@ -340,7 +350,7 @@ def run_accessor(C, args):
     import numpy as np
     wl = dict(WORKLOADS["c2"])
     wl.pop("loops")
-    policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
+    policy = policy_of(C, args.policy)
     params = C.GenParams(**wl, profile=C.PROFILE_REFERENCE)
     host = C.generate(params)
     N, W = len(host.nodes), len(host.wires)
@@ -443,7 +453,7 @@ def main():
     if args.accessor:
         return run_accessor(C, args)
 
-    policy = {"exact": C.POLICY_EXACT, "recip": C.POLICY_RECIP, "fast": C.POLICY_FAST}[args.policy]
+    policy = policy_of(C, args.policy)
     mode = {"atomic": C.REDUCE_ATOMIC, "csr": C.REDUCE_CSR,
             "kahan64": C.REDUCE_CSR_KAHAN64}[args.mode]
     wl = dict(WORKLOADS[args.workload])
@@ -493,13 +503,14 @@ def main():
                      "flops_per_launch": FLOPS_PER_WIRE_STEP * C.STEPS * W,
                      "avg_launch_ms": cnc_ms, "launches": cnc_n},
     }
-    if args.policy == "fast":
-        # FP32 work the FAST step executes: 20 FMA + 8 add per wire-step (the
-        # kernel's instruction mix; the VALU counters agree), against the
-        # algorithmic 96 of the reference formula above.
-        executed = EXECUTED_FLOPS_FAST * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
+    if args.policy in EXECUTED_FLOPS:
+        # FP32 work the step executes per wire-step (the kernel's instruction
+        # mix; the VALU counters agree), against the algorithmic 96 of the
+        # reference formula above.
+        ef = EXECUTED_FLOPS[args.policy]
+        executed = ef * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
         out["roofline"]["executed"] = {
-            "flops_per_wire_step": EXECUTED_FLOPS_FAST, "tflops": executed,
+            "flops_per_wire_step": ef, "tflops": executed,
             "frac": executed / PEAK_FP32_TFLOPS}
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
@@ -515,7 +526,7 @@ def main():
     if not args.no_extra:
         pol = {}
         for name, pv in (("exact", C.POLICY_EXACT), ("recip", C.POLICY_RECIP),
-                         ("fast", C.POLICY_FAST)):
+                         ("fast", C.POLICY_FAST), ("scaled", C.POLICY_SCALED)):
             loops = 2 if pv == C.POLICY_EXACT else 4
             r2, w2, k2 = time_loops(C, ctx, loops, pv, mode)
             ms = k2[C.CALC_NEW_CURRENTS][0]

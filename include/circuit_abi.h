@@ -115,8 +115,15 @@ enum {
  *           step's current update (a tridiagonal update of the 10 currents per
  *           step: 20 FMAs and 8 adds).  Every step's currents are computed; internal
  *           voltages are formed from the final currents.
- * RECIP and FAST are not bit-exact; their error is checked against 1e-5. */
-enum { CKT_POLICY_EXACT = 0, CKT_POLICY_RECIP = 1, CKT_POLICY_FAST = 2 };
+ *   SCALED: FAST's update iterated as the deviation from its fixed point (one
+ *           tridiagonal solve per wire and loop), scaled by the coupling
+ *           coefficient so that each step is 8 adds and 10 FMAs, plus 10
+ *           multiplies per 4 steps.  Used for wires shown to contract or to
+ *           diverge; other wires take FAST's step.
+ * RECIP, FAST and SCALED are not bit-exact; their error is checked against
+ * 1e-5.  FAST and SCALED write a wire whose currents left the finite range as
+ * all NaN, which is where the reference recurrence ends such a wire. */
+enum { CKT_POLICY_EXACT = 0, CKT_POLICY_RECIP = 1, CKT_POLICY_FAST = 2, CKT_POLICY_SCALED = 3 };
 
 /* Reduction mode of distribute_charge.
  *   ATOMIC      : float atomics (LDS-privatised per piece when the piece's

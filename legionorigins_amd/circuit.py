@@ -29,7 +29,7 @@ PRIVATE_PTR, SHARED_PTR, GHOST_PTR = 0, 1, 2
  DISTRIBUTE_CHARGE_NOP, UPDATE_VOLTAGES_NOP, SANITY_CHECK_WIRES) = range(8)
 REDUCE_ID = 1
 
-POLICY_EXACT, POLICY_RECIP, POLICY_FAST = 0, 1, 2
+POLICY_EXACT, POLICY_RECIP, POLICY_FAST, POLICY_SCALED = 0, 1, 2, 3
 REDUCE_ATOMIC, REDUCE_CSR, REDUCE_CSR_KAHAN64 = 0, 1, 2
 PROFILE_REFERENCE, PROFILE_STABLE = 0, 1
 (OPT_STEPS, OPT_CNC_BLOCK, OPT_PERSISTENT, OPT_WAVES_PER_SIMD, OPT_DSC_ATOMIC_IMPL,

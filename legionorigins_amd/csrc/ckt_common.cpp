@@ -148,7 +148,7 @@ int ckt_calc_new_currents_gpu(const ckt_piece_t* piece, ckt_wire_t* wires, const
                               void* stream) {
   if (!piece || !wires || !pvt || !owned || !ghost)
     return fail(CKT_ERR_INVALID_ARG, "ckt_calc_new_currents_gpu: null argument");
-  if (flag < CKT_POLICY_EXACT || flag > CKT_POLICY_FAST)
+  if (flag < CKT_POLICY_EXACT || flag > CKT_POLICY_SCALED)
     return fail(CKT_ERR_INVALID_ARG, "ckt_calc_new_currents_gpu: flag is not a CKT_POLICY_*");
   int st;
   unsigned long long* bad = bad_counter(&st);

@@ -686,7 +686,7 @@ void Engine::reset_timing() {
 }
 
 int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
-  if (policy < CKT_POLICY_EXACT || policy > CKT_POLICY_FAST)
+  if (policy < CKT_POLICY_EXACT || policy > CKT_POLICY_SCALED)
     return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: unknown policy");
   int st;
   if ((st = ensure_table())) return st;

@@ -4,7 +4,7 @@
 //
 //   ckt_sim [-l loops] [-p pieces] [-npp nodes/piece] [-wpp wires/piece]
 //           [-pct pct_in_piece] [-s seed] [-sync 0|1|2]
-//           [-policy exact|recip|fast] [-mode atomic|csr|kahan64]
+//           [-policy exact|recip|fast|scaled] [-mode atomic|csr|kahan64]
 //           [-profile reference|stable] [-graph] [-per_piece] [-device d]
 //           [-devgen]   (generate the circuit in HBM instead of on the host)
 #include <cstdio>
@@ -45,8 +45,10 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "-devgen")) devgen = 1;
     else if (!std::strcmp(argv[i], "-policy")) {
       const char* s = next_str();
-      policy = !std::strcmp(s, "fast") ? CKT_POLICY_FAST
-               : !std::strcmp(s, "recip") ? CKT_POLICY_RECIP : CKT_POLICY_EXACT;
+      policy = !std::strcmp(s, "scaled") ? CKT_POLICY_SCALED
+               : !std::strcmp(s, "fast")   ? CKT_POLICY_FAST
+               : !std::strcmp(s, "recip")  ? CKT_POLICY_RECIP
+                                           : CKT_POLICY_EXACT;
     } else if (!std::strcmp(argv[i], "-mode")) {
       const char* s = next_str();
       mode = !std::strcmp(s, "csr") ? CKT_REDUCE_CSR

@@ -165,7 +165,7 @@ int ckt_server_create(int device, const ckt_server_config_t* config, ckt_server_
   *out = nullptr;
   ckt_server_config_t cfg{};
   if (config) cfg = *config;
-  if (cfg.policy < CKT_POLICY_EXACT || cfg.policy > CKT_POLICY_FAST)
+  if (cfg.policy < CKT_POLICY_EXACT || cfg.policy > CKT_POLICY_SCALED)
     return fail(CKT_ERR_INVALID_ARG, "server: unknown policy");
   if (cfg.reduce_mode < CKT_REDUCE_ATOMIC || cfg.reduce_mode > CKT_REDUCE_CSR_KAHAN64)
     return fail(CKT_ERR_INVALID_ARG, "server: unknown reduction mode");

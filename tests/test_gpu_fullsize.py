@@ -73,7 +73,8 @@ def test_c2_atomic_and_csr_agree(C):
     assert max_rel_err(out[C.REDUCE_ATOMIC]["voltage"], out[C.REDUCE_CSR]["voltage"]) <= TOL
 
 
-def test_c2_fast_sampled_wires_within_tolerance(C, O):
+@pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
+def test_c2_fast_sampled_wires_within_tolerance(C, O, policy):
     # The benchmark's configuration (FAST, fused ATOMIC reduction) at full
     # size, stable profile: one loop, a random sample of wires against the
     # oracle's reference-order relaxation (per wire, relative to the wire's
@@ -83,7 +84,7 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O):
     ref = O.generate(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
     with C.Context(0) as ctx:
         ctx.generate_device(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE))
-        ctx.run_loops(1, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC)
+        ctx.run_loops(1, policy=policy, mode=C.REDUCE_ATOMIC)
         nodes, wires = ctx.store()
     rng = np.random.default_rng(1)
     sample = np.sort(rng.choice(len(ref.wires), 1500, replace=False))
@@ -103,7 +104,8 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O):
     assert max_rel_err(nodes["voltage"], want) <= TOL
 
 
-def test_c2_fast_reference_profile_classes_after_one_loop(C, O):
+@pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
+def test_c2_fast_reference_profile_classes_after_one_loop(C, O, policy):
     # The benchmark's configuration and profile (reference parameters, FAST,
     # fused ATOMIC), one loop: every sampled wire's currents and internal
     # voltages have the oracle's non-finite classes, and every node voltage
@@ -111,11 +113,11 @@ def test_c2_fast_reference_profile_classes_after_one_loop(C, O):
     P, npp, wpp, pct, seed = C2
     gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_REFERENCE)
     got = {}
-    for name, policy, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
-                               ("fast+atomic", C.POLICY_FAST, C.REDUCE_ATOMIC)):
+    for name, pol, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
+                            ("fast+atomic", policy, C.REDUCE_ATOMIC)):
         with C.Context(0) as ctx:
             ctx.generate_device(gp)
-            ctx.run_loops(1, policy=policy, mode=mode)
+            ctx.run_loops(1, policy=pol, mode=mode)
             got[name] = ctx.store()
     ref = O.generate(P, npp, wpp, pct, seed, O.PROFILE_REFERENCE)
     rng = np.random.default_rng(2)
@@ -141,7 +143,8 @@ def test_c2_ten_loops_against_exact_and_fp64(C, O):
     gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
     out = {}
     for name, policy, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
-                               ("fast+atomic", C.POLICY_FAST, C.REDUCE_ATOMIC)):
+                               ("fast+atomic", C.POLICY_FAST, C.REDUCE_ATOMIC),
+                               ("scaled+atomic", C.POLICY_SCALED, C.REDUCE_ATOMIC)):
         with C.Context(0) as ctx:
             ctx.generate_device(gp)
             ctx.run_loops(10, policy=policy, mode=mode, use_graph=True)
@@ -150,9 +153,10 @@ def test_c2_ten_loops_against_exact_and_fp64(C, O):
     truth, _ = O.run_f64_limit(O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE), 10)
     tv = truth["voltage"]
     errs = {k: max_rel_err(v, tv) for k, v in out.items()}
-    diff = max_rel_err(out["fast+atomic"], out["exact+csr"])
+    diff = {k: max_rel_err(out[k], out["exact+csr"]) for k in ("fast+atomic", "scaled+atomic")}
     print(f"\nC2, 10 loops, stable profile: max relative node-voltage error vs FP64 "
-          f"exact+csr {errs['exact+csr']:.3e}, fast+atomic {errs['fast+atomic']:.3e}; "
-          f"fast+atomic vs exact+csr {diff:.3e}")
-    assert diff <= TOL
+          f"exact+csr {errs['exact+csr']:.3e}, fast+atomic {errs['fast+atomic']:.3e}, "
+          f"scaled+atomic {errs['scaled+atomic']:.3e}; vs exact+csr: fast+atomic "
+          f"{diff['fast+atomic']:.3e}, scaled+atomic {diff['scaled+atomic']:.3e}")
+    assert max(diff.values()) <= TOL
     assert max(errs.values()) <= 1e-3

@@ -75,7 +75,7 @@ def test_c1_kahan64_bit_identical(C, O):
     assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
 
 
-@pytest.mark.parametrize("policy", [0, 1, 2], ids=["exact", "recip", "fast"])
+@pytest.mark.parametrize("policy", [0, 1, 2, 3], ids=["exact", "recip", "fast", "scaled"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "csr"])
 def test_c1_stable_policies_within_tolerance(C, O, policy, mode):
     nodes, wires = run_gpu(C, **C1, profile=1, loops=5, policy=policy, mode=mode)
@@ -91,7 +91,7 @@ def test_c1_stable_policies_within_tolerance(C, O, policy, mode):
 
 
 @pytest.mark.parametrize("loops", [1, 5])
-@pytest.mark.parametrize("policy", [1, 2], ids=["recip", "fast"])
+@pytest.mark.parametrize("policy", [1, 2, 3], ids=["recip", "fast", "scaled"])
 def test_c1_reference_profile_fast_policies_match_nan_classes(C, O, policy, loops):
     # After one loop the currents carry the divergence themselves (the node
     # voltages are still finite going in); after five, NaN node voltages
@@ -170,7 +170,7 @@ def test_one_inner_step_is_ohms_law_on_gpu(C):
         ctx.generate(C.GenParams(2, 64, 256, 80, 5, C.PROFILE_STABLE))
         ctx.set_option(C.OPT_STEPS, 1)
         n0, w0 = ctx.store()
-        for policy in (C.POLICY_EXACT, C.POLICY_RECIP, C.POLICY_FAST):
+        for policy in (C.POLICY_EXACT, C.POLICY_RECIP, C.POLICY_FAST, C.POLICY_SCALED):
             ctx.generate(C.GenParams(2, 64, 256, 80, 5, C.PROFILE_STABLE))
             ctx.calc_new_currents(None, policy)
             _, w = ctx.store()
@@ -241,7 +241,7 @@ def test_accessor_form_dropins(C, O):
         C.calc_new_currents_gpu(c.pieces[0], dw, pvt, owned, ghost)
 
 
-@pytest.mark.parametrize("flag", [1, 2], ids=["recip", "fast"])
+@pytest.mark.parametrize("flag", [1, 2, 3], ids=["recip", "fast", "scaled"])
 def test_accessor_form_fast_policies(C, O, flag):
     c = O.generate(4, 256, 1024, 70, 32, O.PROFILE_STABLE)
     dw = C.DeviceArray(c.wires)
