@@ -1,6 +1,6 @@
 """Precision study (C1 circuit, 200 loops, stable profile, pct 95 and 50).
 
-For each policy (EXACT, FAST) and reduction (ATOMIC, CSR, CSR_KAHAN64) the
+For each policy (EXACT, FAST, SCALED) and reduction (ATOMIC, CSR, CSR_KAHAN64) the
 GPU's node voltages after 200 loops are compared with the FP64 oracle (the
 relaxation's FP64 limit, pinned against the step-by-step FP64 relaxation in
 tests/test_oracle_physics.py).  The FP64 error is reported per configuration
@@ -26,7 +26,8 @@ def test_c3_precision_study(C, O, pct):
     shared = O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE).locator == 0
     rows = {}
     ref32 = None
-    for policy, pname in ((C.POLICY_EXACT, "exact"), (C.POLICY_FAST, "fast")):
+    for policy, pname in ((C.POLICY_EXACT, "exact"), (C.POLICY_FAST, "fast"),
+                          (C.POLICY_SCALED, "scaled")):
         for mode, mname in ((C.REDUCE_CSR, "csr"), (C.REDUCE_ATOMIC, "atomic"),
                             (C.REDUCE_CSR_KAHAN64, "csr_kahan64")):
             with C.Context(0) as ctx:
