@@ -1,5 +1,6 @@
 """Batching on the GPU: circuits of uneven size advanced by shared launches
-give the same state as each circuit run alone (EXACT + CSR: bit-identical)."""
+give the same state as each circuit run alone (EXACT or SCALED with CSR:
+bit-identical; ATOMIC: within 1e-5)."""
 import numpy as np
 import pytest
 
@@ -27,11 +28,14 @@ def alone(C, shape, policy, mode):
 
 @pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("persistent", [0, 1])
-def test_batch_equals_each_circuit_alone(C, use_graph, persistent):
+@pytest.mark.parametrize("policy", [0, 3], ids=["exact", "scaled"])
+def test_batch_equals_each_circuit_alone(C, use_graph, persistent, policy):
+    # With the deterministic CSR reduction a batched circuit is bit-identical
+    # to the same circuit run alone under the same policy.
     b = C.Batch(0)
     ids = [b.add_generated(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE), loops)
            for (P, npp, wpp, pct, seed, loops) in SHAPES]
-    cfg = C.Batch.config(policy=C.POLICY_EXACT, mode=C.REDUCE_CSR, use_graph=use_graph,
+    cfg = C.Batch.config(policy=policy, mode=C.REDUCE_CSR, use_graph=use_graph,
                          block_threads=128 if persistent else 256, persistent=persistent,
                          waves_per_simd=4)
     rep = b.run(cfg)
@@ -41,17 +45,18 @@ def test_batch_equals_each_circuit_alone(C, use_graph, persistent):
         P, npp, wpp, *_ = shape
         assert b.loops_left(cid) == 0
         nodes, wires = b.store(cid, P * npp, P * wpp)
-        n1, w1 = alone(C, shape, C.POLICY_EXACT, C.REDUCE_CSR)
+        n1, w1 = alone(C, shape, policy, C.REDUCE_CSR)
         assert nodes.tobytes() == n1.tobytes()
         assert wires.tobytes() == w1.tobytes()
     b.close()
 
 
-def test_batch_fast_atomic_within_tolerance(C):
+@pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
+def test_batch_fast_atomic_within_tolerance(C, policy):
     b = C.Batch(0)
     ids = [b.add_generated(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE), loops)
            for (P, npp, wpp, pct, seed, loops) in SHAPES]
-    b.run(C.Batch.config(policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC))
+    b.run(C.Batch.config(policy=policy, mode=C.REDUCE_ATOMIC))
     for cid, shape in zip(ids, SHAPES):
         P, npp, wpp, *_ = shape
         nodes, _ = b.store(cid, P * npp, P * wpp)
