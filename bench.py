@@ -294,8 +294,26 @@ def run_upv_sweep(C, args):
                      "frac_of_8TBs": gbs / PEAK_HBM_GBS, "bytes_per_launch": 24 * N})
         log(f"[upv] {rows[-1]}")
         ctx.close()
+    # Reference point on the same GPU: the library's plain copy kernel
+    # (128-bit ordinary loads and stores) over 1 GiB, 20 back-to-back launches
+    # timed around one synchronisation.
+    import numpy as np
+    nbytes, reps = 1 << 30, 20
+    src = C.DeviceArray(np.zeros(nbytes // 4, np.float32))
+    dst = C.DeviceArray(np.zeros(nbytes // 4, np.float32))
+    C.copy_kernel(dst, src, nbytes)
+    C.lib().ckt_device_synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        C.copy_kernel(dst, src, nbytes)
+    C.lib().ckt_device_synchronize(0)
+    copy_ms = (time.perf_counter() - t0) / reps * 1e3
+    copy_gbs = 2.0 * nbytes / (copy_ms * 1e-3) / 1e9
+    copy = {"bytes_moved": 2 * nbytes, "avg_launch_ms": copy_ms, "achieved_GBs": copy_gbs,
+            "frac_of_8TBs": copy_gbs / PEAK_HBM_GBS}
+    log(f"[upv] plain copy: {copy}")
     print(json.dumps({"metric": "update_voltages HBM bandwidth", "unit": "GB/s",
-                      "algorithmic_bytes_per_node": 24, "rows": rows}))
+                      "algorithmic_bytes_per_node": 24, "rows": rows, "plain_copy_1GiB": copy}))
 
 
 def run_dsc_sweep(C, args):

@@ -26,7 +26,8 @@ def test_driver_without_gpu_reports_error():
 @pytest.mark.gpu
 @pytest.mark.parametrize("extra", [[], ["-sync", "1"], ["-policy", "fast", "-graph"],
                                    ["-mode", "csr", "-per_piece", "-profile", "stable"],
-                                   ["-devgen", "-policy", "fast"]])
+                                   ["-devgen", "-policy", "fast"],
+                                   ["-devgen", "-policy", "scaled", "-graph"]])
 def test_driver_report_format(extra):
     r = run("-l", "2", "-p", "4", "-npp", "1024", "-wpp", "4096", "-pct", "95", "-s", "12345",
             *extra)

@@ -77,3 +77,36 @@ def test_scaled_short_step_counts(C, steps):
         assert max_wire_rel_err(wires["current"], ew["current"]) <= TOL, name
         assert max_wire_rel_err(wires["voltage"], ew["voltage"]) <= TOL, name
         assert max_rel_err(nodes["voltage"], en["voltage"]) <= TOL, name
+
+
+def test_scaled_matches_fast_on_random_parameters(C, O):
+    # Wire parameters and states spread over many decades, so that
+    # g = L/(dt R) runs over 1e-8..1e8 and d = dt/(R C) over 1e-11..1e13:
+    # contracting, diverging, marginal and tiny-coupling wires side by side,
+    # mixed within waves.  SCALED must give FAST's non-finite class on every
+    # element and FAST's finite values to 1e-5 (per wire, relative to the
+    # wire's largest segment value), bit for bit where it falls back.
+    c = O.generate(4, 256, 1024, 95, 41, O.PROFILE_STABLE)
+    w, n = c.wires, c.nodes
+    rng = np.random.default_rng(11)
+    W = len(w)
+    w["resistance"] = (10.0 ** rng.uniform(-2, 2, W)).astype(np.float32)
+    w["inductance"] = (10.0 ** rng.uniform(-12, 0, W)).astype(np.float32)
+    w["capacitance"] = (10.0 ** rng.uniform(-9, 3, W)).astype(np.float32)
+    w["current"] = (rng.uniform(-1, 1, (W, 10)) * 10.0 ** rng.uniform(-3, 3, (W, 1))).astype(np.float32)
+    w["voltage"] = (rng.uniform(-1, 1, (W, 9)) * 10.0 ** rng.uniform(-2, 2, (W, 1))).astype(np.float32)
+    n["voltage"] = (rng.uniform(-1, 1, len(n)) * 10.0 ** rng.uniform(-2, 2, len(n))).astype(np.float32)
+    scaled = _run_accessor(C, c, C.POLICY_SCALED)
+    fast = _run_accessor(C, c, C.POLICY_FAST)
+    for f in ("current", "voltage"):
+        assert_same_class(scaled[f], fast[f], f)
+        fin = np.isfinite(fast[f]).all(axis=1)
+        assert max_wire_rel_err(scaled[f][fin], fast[f][fin]) <= TOL, f
+    # the regimes are all present
+    dt = 1e-6
+    R, L, Cw = (w[k].astype(np.float64) for k in ("resistance", "inductance", "capacitance"))
+    g, d = L / dt / R, dt / (R * Cw)
+    assert (g - 4 * d >= 2).sum() > 100 and (np.maximum(np.abs(2 * d - g) + 2 * d, 0) <= 0.5).sum() > 100
+    assert (d < 2.0 ** -25).sum() > 100
+    nonfin = ~np.isfinite(fast["current"]).all(axis=1)
+    assert 0 < nonfin.sum() < W
