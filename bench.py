@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="untimed loops")
     ap.add_argument("--policy", default="scaled", choices=["exact", "recip", "fast", "scaled"])
     ap.add_argument("--mode", default="atomic", choices=["atomic", "csr", "kahan64"])
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="64x16384x65536",
+                    choices=["4x1024x4096", "64x16384x65536"],
+                    help="pieces x nodes/piece x wires/piece")
     ap.add_argument("--profile", default="reference", choices=["reference", "stable"],
                     help="generator profile of the timed circuit (stable: finite values)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -65,7 +67,7 @@ def parse():
     ap.add_argument("--dsc-sweep", action="store_true",
                     help="distribute_charge bandwidth per reduction mode at growing sizes")
     ap.add_argument("--accessor", action="store_true",
-                    help="the drop-in accessor form at C2: per-piece synchronous calls on "
+                    help="the drop-in accessor form at 64x16384x65536: per-piece synchronous calls on "
                          "reference-layout instances")
     ap.add_argument("--gen", action="store_true",
                     help="circuit generation: host generator + upload vs the device generator")
@@ -75,10 +77,10 @@ def parse():
 
 
 WORKLOADS = {
-    "c1": dict(num_pieces=4, nodes_per_piece=1024, wires_per_piece=4096, pct_wire_in_piece=95,
-               random_seed=12345, loops=5),
-    "c2": dict(num_pieces=64, nodes_per_piece=16384, wires_per_piece=65536, pct_wire_in_piece=95,
-               random_seed=12345, loops=10),
+    "4x1024x4096": dict(num_pieces=4, nodes_per_piece=1024, wires_per_piece=4096,
+                        pct_wire_in_piece=95, random_seed=12345, loops=5),
+    "64x16384x65536": dict(num_pieces=64, nodes_per_piece=16384, wires_per_piece=65536,
+                           pct_wire_in_piece=95, random_seed=12345, loops=10),
 }
 
 
@@ -357,16 +359,16 @@ def run_dsc_sweep(C, args):
 
 def run_accessor(C, args):
     """The three tasks through the accessor-form entry points -- what the
-    reference's GPU task wrappers call (circuit.cc:475-550) -- on C2 in the
-    reference layouts (128-byte wires, 16-byte nodes) in device memory: one
-    call per piece per task, each waiting for its kernel, as the reference
-    runtime runs GPU point tasks.  On one GPU the private, shared and ghost
+    reference's GPU task wrappers call (circuit.cc:475-550) -- on the 64-piece
+    circuit (4.19 M wires) in the reference layouts (128-byte wires, 16-byte
+    nodes) in device memory: one call per piece per task, each waiting for its
+    kernel, as the reference runtime runs GPU point tasks.  On one GPU the private, shared and ghost
     node instances are one array; shared and ghost charges go to fold buffers
     that are applied once per loop (the runtime's reduction-instance apply).
     Beside it, the resident context in its per-piece mode and in its normal
     one-launch-per-task mode, on the same circuit."""
     import numpy as np
-    wl = dict(WORKLOADS["c2"])
+    wl = dict(WORKLOADS["64x16384x65536"])
     wl.pop("loops")
     policy = policy_of(C, args.policy)
     params = C.GenParams(**wl, profile=C.PROFILE_REFERENCE)
@@ -411,7 +413,7 @@ def run_accessor(C, args):
         loops, policy=policy, mode=C.REDUCE_ATOMIC)["wire_updates_per_s"]
     ctx.close()
     log(f"[accessor] {rows}")
-    print(json.dumps({"metric": "wire-updates/s at C2 by calling form", "unit": "wire-updates/s",
+    print(json.dumps({"metric": "wire-updates/s at 64x16384x65536 by calling form", "unit": "wire-updates/s",
                       "policy": args.policy, "loops": loops, "rows": rows}))
 
 

@@ -1,11 +1,11 @@
-"""Full-size bit-identity check, too slow for the test suite: the C2 circuit
+"""Full-size bit-identity check, too slow for the test suite: the full-size circuit
 (64 pieces x 16384 nodes x 65536 wires, 95% in-piece, seed 12345) for its 10
 loops with EXACT + CSR on the GPU against the oracle's FP32 restatement of the
 reference tasks (circuit.cc:352-460, sequential reduction order) on the host,
 every node and wire compared bit for bit.  Both profiles.  The oracle needs
 about 1.7 core-hours per profile (~7 minutes on 16 threads).
 
-    python tests/c2_exact_vs_oracle.py [--profile reference|stable] [--loops N]
+    python tests/full_size_exact_vs_oracle.py [--profile reference|stable] [--loops N]
 """
 import argparse
 import os
@@ -35,18 +35,18 @@ def main():
         ctx.generate(C.GenParams(P, npp, wpp, pct, seed, prof))
         ctx.run_loops(a.loops, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
         nodes, wires = ctx.store()
-    print(f"[c2] GPU EXACT+CSR {a.loops} loops done in {time.time() - t0:.1f} s", flush=True)
+    print(f"[full size] GPU EXACT+CSR {a.loops} loops done in {time.time() - t0:.1f} s", flush=True)
     ref = O.generate(P, npp, wpp, pct, seed, prof)
     t0 = time.time()
     for loop in range(a.loops):
         O.run(ref, 1, reduce=0)
-        print(f"[c2] oracle loop {loop + 1}/{a.loops} at {time.time() - t0:.0f} s", flush=True)
+        print(f"[full size] oracle loop {loop + 1}/{a.loops} at {time.time() - t0:.0f} s", flush=True)
     assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
     assert_bit_identical(nodes["charge"], ref.nodes["charge"], "node charge")
     assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
     assert_bit_identical(wires["voltage"], ref.wires["voltage"], "wire voltage")
     nan = float(np.isnan(nodes["voltage"]).mean())
-    print(f"[c2] {a.profile} profile, {a.loops} loops: EXACT+CSR bit-identical to the oracle on "
+    print(f"[full size] {a.profile} profile, {a.loops} loops: EXACT+CSR bit-identical to the oracle on "
           f"{len(nodes)} nodes and {len(wires)} wires (NaN fraction of node voltages {nan:.4f})")
 
 

@@ -1,6 +1,6 @@
 """Writes the committed fixtures of tests/golden/ from the CPU oracle.
 
-generator_c1.json : digests of the C1 circuit (4 pieces x 1024 nodes x 4096
+generator_4x1024x4096.json : digests of the 4-piece circuit (4 pieces x 1024 nodes x 4096
                     wires, 95% in-piece, seed 12345, reference profile), its
                     drand48 draw count and its first node records.
 Run from the repository root: python tests/golden/make_golden.py
@@ -26,7 +26,7 @@ def main():
         "draws": int(c.draws),
         "first_nodes": {str(i): [float(x) for x in c.nodes[i].tolist()] for i in range(4)},
     }
-    with open(os.path.join(HERE, "generator_c1.json"), "w") as f:
+    with open(os.path.join(HERE, "generator_4x1024x4096.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
 

@@ -46,8 +46,8 @@ def test_record_layouts_match_reference(C, O):
 
 
 @pytest.mark.parametrize("P,npp,wpp,pct,seed,profile", [
-    (4, 1024, 4096, 95, 12345, 0),  # C1
-    (4, 1024, 4096, 50, 12345, 0),  # C3 at pct 50
+    (4, 1024, 4096, 95, 12345, 0),  # 4 pieces, pct 95
+    (4, 1024, 4096, 50, 12345, 0),  # 4 pieces, pct 50
     (4, 1024, 4096, 95, 12345, 1),  # stable profile
     (1, 300, 1200, 80, 7, 0),       # one piece, cross branch wraps
     (7, 33, 129, 0, -5, 1),         # every wire crosses pieces

@@ -3,11 +3,11 @@ against the oracle's generator and the library's host generator.
 
 ckt_ctx_generate_device must load the same circuit bit for bit as
 ckt_ctx_generate: node and wire fields, locations, the locator and the
-node -> (wire, end) index.  Cases: C1 in both profiles, pct 0 / 50 / 100,
-one piece (the wrap rule), no wires, and a circuit of 12.6 M wires whose
-wire start states cross several host -> device chunks (2^22 wires each).
-A device-generated C1 then runs 5 loops of EXACT + CSR bit-identical to
-the oracle.
+node -> (wire, end) index.  Cases: the 4-piece circuit (4 x 1024 nodes x
+4096 wires) in both profiles, pct 0 / 50 / 100, one piece (the wrap rule), no
+wires, and a circuit of 12.6 M wires whose wire start states cross several
+host -> device chunks (2^22 wires each).  A device-generated 4-piece circuit
+then runs 5 loops of EXACT + CSR bit-identical to the oracle.
 """
 import sys
 import time
@@ -107,7 +107,7 @@ def test_device_generator_several_chunks(C):
 
 def test_device_generated_c1_runs_bit_identical(C, O):
     gp = C.GenParams(4, 1024, 4096, 95, 12345, 1)
-    say("[devgen] C1 stable, 5 loops EXACT + CSR")
+    say("[devgen] 4-piece circuit, stable, 5 loops EXACT + CSR")
     with C.Context(0) as ctx:
         ctx.generate_device(gp)
         ctx.run_loops(5, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)

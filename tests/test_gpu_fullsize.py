@@ -1,4 +1,4 @@
-"""Full-size (C2: 64 pieces x 16384 nodes x 65536 wires, ~4.2M wires) checks
+"""Full-size (64 pieces x 16384 nodes x 65536 wires, ~4.2M wires) checks
 through size-independent properties.
 
   * the generator is bit-identical to the oracle at this size;
@@ -22,12 +22,12 @@ from helpers import (TOL, assert_bit_identical, assert_same_class, max_rel_err,
 
 pytestmark = pytest.mark.gpu
 
-C2 = (64, 16384, 65536, 95, 12345)
+FULL = (64, 16384, 65536, 95, 12345)  # 4,194,304 wires, 1,048,576 nodes
 
 
 @pytest.mark.parametrize("profile", [1, 0], ids=["stable", "reference"])
-def test_c2_one_loop_properties(C, O, profile):
-    P, npp, wpp, pct, seed = C2
+def test_full_one_loop_properties(C, O, profile):
+    P, npp, wpp, pct, seed = FULL
     host = C.generate(C.GenParams(P, npp, wpp, pct, seed, profile))
     ref = O.generate(P, npp, wpp, pct, seed, profile)
     assert host.wires.tobytes() == ref.wires.tobytes()
@@ -61,8 +61,8 @@ def test_c2_one_loop_properties(C, O, profile):
         assert_same_class(nodes["voltage"], want.astype(np.float32), "node voltage class")
 
 
-def test_c2_atomic_and_csr_agree(C):
-    P, npp, wpp, pct, seed = C2
+def test_full_atomic_and_csr_agree(C):
+    P, npp, wpp, pct, seed = FULL
     out = {}
     for mode in (C.REDUCE_ATOMIC, C.REDUCE_CSR):
         with C.Context(0) as ctx:
@@ -74,13 +74,13 @@ def test_c2_atomic_and_csr_agree(C):
 
 
 @pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
-def test_c2_fast_sampled_wires_within_tolerance(C, O, policy):
+def test_full_fast_sampled_wires_within_tolerance(C, O, policy):
     # The benchmark's configuration (FAST, fused ATOMIC reduction) at full
     # size, stable profile: one loop, a random sample of wires against the
     # oracle's reference-order relaxation (per wire, relative to the wire's
     # largest segment value), and every node voltage against the float64
     # scatter-add of the GPU's own currents.
-    P, npp, wpp, pct, seed = C2
+    P, npp, wpp, pct, seed = FULL
     ref = O.generate(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
     with C.Context(0) as ctx:
         ctx.generate_device(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE))
@@ -105,12 +105,12 @@ def test_c2_fast_sampled_wires_within_tolerance(C, O, policy):
 
 
 @pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
-def test_c2_fast_reference_profile_classes_after_one_loop(C, O, policy):
+def test_full_fast_reference_profile_classes_after_one_loop(C, O, policy):
     # The benchmark's configuration and profile (reference parameters, FAST,
     # fused ATOMIC), one loop: every sampled wire's currents and internal
     # voltages have the oracle's non-finite classes, and every node voltage
-    # has the class of EXACT + CSR's (bit-identical to the oracle at C2).
-    P, npp, wpp, pct, seed = C2
+    # has the class of EXACT + CSR's (bit-identical to the oracle at this size).
+    P, npp, wpp, pct, seed = FULL
     gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_REFERENCE)
     got = {}
     for name, pol, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),
@@ -132,14 +132,15 @@ def test_c2_fast_reference_profile_classes_after_one_loop(C, O, policy):
                       "node voltage")
 
 
-def test_c2_ten_loops_against_exact_and_fp64(C, O):
-    # C2 for its stated 10 loops, stable profile.  FAST + fused ATOMIC (the
-    # benchmark's configuration) against EXACT + CSR -- bit-identical to the
-    # CPU restatement at C1/C3, on the sampled C2 wires above and over all of
-    # C2 for 10 loops (tests/c2_exact_vs_oracle.py, ~1.7 core-hours of CPU),
-    # so it stands in for the CPU run -- on every node voltage; both against
-    # the FP64 oracle (the relaxation's FP64 limit), printed with -rP.
-    P, npp, wpp, pct, seed = C2
+def test_full_ten_loops_against_exact_and_fp64(C, O):
+    # The full-size circuit for 10 loops, stable profile.  FAST and SCALED
+    # with the fused ATOMIC reduction (the benchmark's configurations) against
+    # EXACT + CSR -- bit-identical to the CPU restatement on the 4-piece
+    # circuits, on the sampled wires above and over this whole circuit for 10
+    # loops (tests/full_size_exact_vs_oracle.py, ~1.7 core-hours of CPU), so it
+    # stands in for the CPU run -- on every node voltage; all against the FP64
+    # oracle (the relaxation's FP64 limit), printed with -rP.
+    P, npp, wpp, pct, seed = FULL
     gp = C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE)
     out = {}
     for name, policy, mode in (("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR),

@@ -1,7 +1,7 @@
 """GPU parity: the HIP kernels, called through the C ABI, against the oracle.
 
-Sizes: C1 (4 pieces x 1024 nodes x 4096 wires, 95% in-piece, seed 12345, 5
-loops of 10000 inner steps) and its pct-50 variant, in the reference profile
+Sizes: the 4-piece circuit (4 pieces x 1024 nodes x 4096 wires, 95% in-piece,
+seed 12345, 5 loops of 10000 inner steps) and its pct-50 variant, in the reference profile
 (values diverge to inf/NaN, parity by class and bit pattern) and the stable
 profile (finite values).  EXACT + CSR must be bit-identical to the oracle;
 EXACT + KAHAN64 bit-identical to the oracle's Kahan restatement; the other
@@ -17,7 +17,7 @@ from helpers import (TOL, assert_bit_identical, assert_same_class, max_rel_err,
 
 pytestmark = pytest.mark.gpu
 
-C1 = dict(P=4, npp=1024, wpp=4096, pct=95, seed=12345)
+SMALL = dict(P=4, npp=1024, wpp=4096, pct=95, seed=12345)
 
 
 def say(*a):
@@ -47,11 +47,11 @@ def run_oracle(O, P, npp, wpp, pct, seed, profile, loops, reduce=0):
 
 
 @pytest.mark.parametrize("profile", [0, 1], ids=["reference", "stable"])
-def test_c1_exact_csr_bit_identical(C, O, profile):
-    say(f"[c1 exact csr] profile={profile}")
-    nodes, wires = run_gpu(C, **C1, profile=profile, loops=5, policy=C.POLICY_EXACT,
+def test_small_exact_csr_bit_identical(C, O, profile):
+    say(f"[4-piece exact csr] profile={profile}")
+    nodes, wires = run_gpu(C, **SMALL, profile=profile, loops=5, policy=C.POLICY_EXACT,
                            mode=C.REDUCE_CSR)
-    ref = run_oracle(O, **C1, profile=profile, loops=5)
+    ref = run_oracle(O, **SMALL, profile=profile, loops=5)
     assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
     assert_bit_identical(nodes["charge"], ref.nodes["charge"], "node charge")
     assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
@@ -59,31 +59,31 @@ def test_c1_exact_csr_bit_identical(C, O, profile):
     say(f"  NaN fraction of node voltages: {np.isnan(nodes['voltage']).mean():.4f}")
 
 
-def test_c3_pct50_exact_csr_bit_identical(C, O):
-    cfg = dict(C1, pct=50)
+def test_small_pct50_exact_csr_bit_identical(C, O):
+    cfg = dict(SMALL, pct=50)
     nodes, wires = run_gpu(C, **cfg, profile=1, loops=5, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
     ref = run_oracle(O, **cfg, profile=1, loops=5)
     assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
     assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
 
 
-def test_c1_kahan64_bit_identical(C, O):
-    nodes, wires = run_gpu(C, **C1, profile=1, loops=5, policy=C.POLICY_EXACT,
+def test_small_kahan64_bit_identical(C, O):
+    nodes, wires = run_gpu(C, **SMALL, profile=1, loops=5, policy=C.POLICY_EXACT,
                            mode=C.REDUCE_CSR_KAHAN64)
-    ref = run_oracle(O, **C1, profile=1, loops=5, reduce=1)
+    ref = run_oracle(O, **SMALL, profile=1, loops=5, reduce=1)
     assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
     assert_bit_identical(wires["current"], ref.wires["current"], "wire current")
 
 
 @pytest.mark.parametrize("policy", [0, 1, 2, 3], ids=["exact", "recip", "fast", "scaled"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "csr"])
-def test_c1_stable_policies_within_tolerance(C, O, policy, mode):
-    nodes, wires = run_gpu(C, **C1, profile=1, loops=5, policy=policy, mode=mode)
-    ref = run_oracle(O, **C1, profile=1, loops=5)
+def test_small_stable_policies_within_tolerance(C, O, policy, mode):
+    nodes, wires = run_gpu(C, **SMALL, profile=1, loops=5, policy=policy, mode=mode)
+    ref = run_oracle(O, **SMALL, profile=1, loops=5)
     e_v = max_rel_err(nodes["voltage"], ref.nodes["voltage"])
     e_i = max_wire_rel_err(wires["current"], ref.wires["current"])
     e_w = max_wire_rel_err(wires["voltage"], ref.wires["voltage"])
-    say(f"[c1 stable policy={policy} mode={mode}] node V {e_v:.2e} wire I {e_i:.2e} wire V {e_w:.2e}")
+    say(f"[4-piece stable policy={policy} mode={mode}] node V {e_v:.2e} wire I {e_i:.2e} wire V {e_w:.2e}")
     assert np.isfinite(nodes["voltage"]).all()
     assert e_v <= TOL
     assert e_i <= TOL
@@ -92,12 +92,12 @@ def test_c1_stable_policies_within_tolerance(C, O, policy, mode):
 
 @pytest.mark.parametrize("loops", [1, 5])
 @pytest.mark.parametrize("policy", [1, 2, 3], ids=["recip", "fast", "scaled"])
-def test_c1_reference_profile_fast_policies_match_nan_classes(C, O, policy, loops):
+def test_small_reference_profile_fast_policies_match_nan_classes(C, O, policy, loops):
     # After one loop the currents carry the divergence themselves (the node
     # voltages are still finite going in); after five, NaN node voltages
     # feed every wire.
-    nodes, wires = run_gpu(C, **C1, profile=0, loops=loops, policy=policy, mode=C.REDUCE_ATOMIC)
-    ref = run_oracle(O, **C1, profile=0, loops=loops)
+    nodes, wires = run_gpu(C, **SMALL, profile=0, loops=loops, policy=policy, mode=C.REDUCE_ATOMIC)
+    ref = run_oracle(O, **SMALL, profile=0, loops=loops)
     assert_same_class(nodes["voltage"], ref.nodes["voltage"], "node voltage")
     assert_same_class(wires["current"], ref.wires["current"], "wire current")
     assert_same_class(wires["voltage"], ref.wires["voltage"], "wire voltage")

@@ -1,4 +1,5 @@
-"""Precision study (C1 circuit, 200 loops, stable profile, pct 95 and 50).
+"""Precision study (4 pieces x 1024 nodes x 4096 wires, 200 loops, stable profile,
+pct 95 and 50).
 
 For each policy (EXACT, FAST, SCALED) and reduction (ATOMIC, CSR, CSR_KAHAN64) the
 GPU's node voltages after 200 loops are compared with the FP64 oracle (the
@@ -19,7 +20,7 @@ LOOPS = 200
 
 
 @pytest.mark.parametrize("pct", [95, 50])
-def test_c3_precision_study(C, O, pct):
+def test_precision_study_200_loops(C, O, pct):
     P, npp, wpp, seed = 4, 1024, 4096, 12345
     truth_n, _ = O.run_f64_limit(O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE), LOOPS)
     tv = truth_n["voltage"]
@@ -42,7 +43,7 @@ def test_c3_precision_study(C, O, pct):
                 "fp64_err_all": max_rel_err(v, tv),
                 "fp64_err_shared": max_rel_err(v[shared], tv[shared]),
                 "err_vs_fp32_reference": max_rel_err(v, ref32)}
-    print(f"\nC3 precision (pct={pct}, {LOOPS} loops): max relative node-voltage error")
+    print(f"\nprecision study (pct={pct}, {LOOPS} loops): max relative node-voltage error")
     print(f"{'config':22s} {'vs FP64 (all)':>14s} {'vs FP64 (shared)':>17s} {'vs FP32 ref':>12s}")
     for k, r in rows.items():
         print(f"{k:22s} {r['fp64_err_all']:14.3e} {r['fp64_err_shared']:17.3e} "
@@ -81,7 +82,7 @@ def test_reduction_error_in_ulps(C):
     ulp = np.spacing(np.maximum(np.abs(want), np.float32(1e-30)))
     host = C.generate(params)
     shared = host.locator == 0
-    print(f"\nreduction error after one loop (pct 50, C1), max |charge - exact| in ulps of the exact sum")
+    print(f"\nreduction error after one loop (pct 50, 4 pieces), max |charge - exact| in ulps of the exact sum")
     for name, q in out.items():
         e = np.abs(q.astype(np.float64) - exact) / ulp
         print(f"{name:12s} shared {e[shared].max():8.2f}   private {e[~shared].max():8.2f}")

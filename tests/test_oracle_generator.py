@@ -6,8 +6,8 @@ Independent evidence used here:
   * a literal model of load_circuit over Python ``set`` objects (sorted-set
     walk for random_element, erase/insert on the privacy maps, find_location
     by membership) on tiny circuits;
-  * draw-count and classification invariants at the C1 size;
-  * the committed digest of the C1 circuit (tests/golden/generator_c1.json).
+  * draw-count and classification invariants at the 4-piece size (4 x 1024 nodes x 4096 wires);
+  * the committed digest of that circuit (tests/golden/generator_4x1024x4096.json).
 """
 import hashlib
 import json
@@ -167,7 +167,7 @@ def _classification_invariants(c, P, npp, wpp):
     return cross
 
 
-def test_c1_draw_count_and_classification(O):
+def test_small_draw_count_and_classification(O):
     P, npp, wpp = 4, 1024, 4096
     c = O.generate(P, npp, wpp, 95, 12345)
     cross = _classification_invariants(c, P, npp, wpp)
@@ -211,8 +211,8 @@ def _digest(c):
     return h
 
 
-def test_c1_golden_digest(O):
-    with open(os.path.join(GOLDEN, "generator_c1.json")) as f:
+def test_small_golden_digest(O):
+    with open(os.path.join(GOLDEN, "generator_4x1024x4096.json")) as f:
         g = json.load(f)
     c = O.generate(4, 1024, 4096, 95, 12345)
     assert _digest(c) == g["sha256"]
