@@ -201,8 +201,8 @@ class Engine {
   unsigned long long* d_bad_ = nullptr;
   int steps_ = CKT_STEPS;
   int cnc_block_ = 256;
-  int persistent_ = 0;
-  int waves_per_simd_ = 8;
+  int persistent_ = -1;      // -1: chosen per policy (launch_shape)
+  int waves_per_simd_ = -1;  // -1: chosen per policy
   int dsc_atomic_impl_ = 0;
   int fuse_dsc_ = 1;
   bool timing_ = false;
