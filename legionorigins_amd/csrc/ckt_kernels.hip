@@ -34,10 +34,16 @@ struct Relax;
 // within 2^-52 of a / b, so both round the same way (the double-rounding
 // argument of S. Figueroa, "When is double rounding innocuous?", 1995).  Zero
 // and subnormal quotients take the IEEE division instead.  Infinite, NaN and
-// zero operands give what the IEEE division gives.
+// zero operands give what the IEEE division gives.  One compare guards the
+// rare path (|q| < 2^-126 holds for zero and subnormal q whether or not the
+// compare treats subnormal inputs as zero); a zero dividend is sorted out
+// inside it.
 __device__ __forceinline__ float div_fixed(float a, double rb, float b) {
   float q = (float)((double)a * rb);
-  if (__builtin_expect(fabsf(q) < 0x1p-126f && a != 0.f, 0)) q = a / b;
+  if (__builtin_expect(fabsf(q) < 0x1p-126f, 0)) {
+    asm volatile("");
+    if (a != 0.f) q = a / b;
+  }
   return q;
 }
 

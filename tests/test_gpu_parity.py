@@ -187,6 +187,31 @@ def test_one_inner_step_is_ohms_law_on_gpu(C):
             assert not w["current"][:, 1:9].any()
 
 
+def test_exact_subnormal_quotients_bit_identical(C, O):
+    # Currents and voltages around 1e-38..1e-44 keep the relaxation in the
+    # subnormal range, so EXACT's divisions return subnormal and zero
+    # quotients (from nonzero and zero dividends): the correctly rounded
+    # division's rare path, bit for bit against the oracle's IEEE division.
+    c = O.generate(2, 256, 1024, 95, 17, O.PROFILE_STABLE)
+    rng = np.random.default_rng(5)
+    W, N = len(c.wires), len(c.nodes)
+    c.wires["current"] = (rng.uniform(-1, 1, (W, 10)) * 1e-38).astype(np.float32)
+    c.wires["voltage"] = (rng.uniform(-1, 1, (W, 9)) * 1e-38).astype(np.float32)
+    c.nodes["voltage"] = (rng.uniform(-1, 1, N) * 1e-38).astype(np.float32)
+    ref = c.copy()
+    dw = C.DeviceArray(c.wires)
+    pvt, owned, ghost = (C.DeviceArray(c.nodes) for _ in range(3))
+    for p in c.pieces:
+        C.calc_new_currents_gpu(p, dw, pvt, owned, ghost, flag=C.POLICY_EXACT)
+        O.calc_new_currents(ref, p)
+    got = dw.download()
+    assert_bit_identical(got["current"], ref.wires["current"], "wire current")
+    assert_bit_identical(got["voltage"], ref.wires["voltage"], "wire voltage")
+    cur = ref.wires["current"]
+    sub = (cur != 0) & (np.abs(cur) < np.float32(2.0 ** -126))
+    assert sub.mean() > 0.5
+
+
 def test_accessor_form_dropins(C, O):
     """The _gpu entry points on reference-layout instances (separate pvt,
     owned and ghost instances, fold buffers, int locator)."""
