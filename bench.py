@@ -6,6 +6,9 @@ their piece, seed 12345, WIRE_SEGMENTS=10, 10000 inner steps per loop, FP32,
 structure-of-arrays in HBM.  One bench step is one loop of the three tasks
 (calc_new_currents, distribute_charge, update_voltages) over all pieces; the
 metric is wire-updates/s = wires x loops / wall time, inputs resident in HBM.
+Policy (default): SCALED, with the ATOMIC reduction fused into
+calc_new_currents; --policy selects EXACT (bit-identical to the oracle), RECIP
+or FAST, and `detail.policies` times all four.
 
 Printed: one JSON line (rank 0) with the metric, a `roofline` object for the
 dominant kernel (calc_new_currents, FP32 VALU-bound) from HIP-event timing on
