@@ -1,7 +1,6 @@
 """Batching on the GPU: circuits of uneven size advanced by shared launches
 give the same state as each circuit run alone (EXACT or SCALED with CSR:
 bit-identical; ATOMIC: within 1e-5)."""
-import numpy as np
 import pytest
 
 from helpers import TOL, assert_bit_identical, max_rel_err

@@ -17,7 +17,6 @@ update_voltages (circuit.cc:1290-1307)
 import math
 
 import numpy as np
-import pytest
 
 f32 = np.float32
 DT32 = f32(1e-6)
