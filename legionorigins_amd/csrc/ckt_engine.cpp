@@ -391,7 +391,8 @@ int Engine::remove(uint32_t id) { return remove_many(&id, 1); }
 // everything already queued on the stream, so removing only such circuits
 // needs no synchronisation; any other slab is freed after one stream
 // synchronisation for the whole set.  The device descriptor table is only
-// rewritten by ensure_table, which waits for the stream first.  Cached tile
+// rewritten by ensure_table, by a copy queued behind every launch already on
+// the stream.  Cached tile
 // lists and graphs are keyed by the engine generation, which changes here, so
 // entries naming the removed circuits can never be looked up again; they stay
 // until the cache's size bound evicts them (drop_caches waits for the stream
@@ -532,19 +533,29 @@ int Engine::download_voltages(uint32_t id, std::vector<float>& v) {
   return CKT_OK;
 }
 
+// The table is rewritten by a copy queued on the stream: it runs after every
+// launch already queued there (which read the old contents) and before every
+// later one.  Graphs are keyed by the engine generation, which changes with
+// the table, so no captured graph is replayed against a rewritten table.
+// Only growing the buffer waits for the stream.
 int Engine::ensure_table() {
   if (!table_dirty_) return CKT_OK;
-  CKT_HIP(hipStreamSynchronize(stream_));
   std::vector<DevCircuit> tab(slots_.size());
   for (size_t i = 0; i < slots_.size(); i++) tab[i] = slots_[i]->dev;
   if (tab.size() > table_cap_) {
+    CKT_HIP(hipStreamSynchronize(stream_));
     if (d_table_) CKT_HIP(hipFree(d_table_));
     d_table_ = nullptr;
     table_cap_ = std::max<size_t>(tab.size(), 2 * table_cap_);
     CKT_HIP(hipMalloc((void**)&d_table_, table_cap_ * sizeof(DevCircuit)));
   }
-  if (!tab.empty())
-    CKT_HIP(hipMemcpy(d_table_, tab.data(), tab.size() * sizeof(DevCircuit), hipMemcpyHostToDevice));
+  if (!tab.empty()) {
+    void* staged = nullptr;
+    int st = stage(tab.data(), tab.size() * sizeof(DevCircuit), &staged);
+    if (st) return st;
+    CKT_HIP(hipMemcpyAsync(d_table_, staged, tab.size() * sizeof(DevCircuit),
+                           hipMemcpyHostToDevice, stream_));
+  }
   table_dirty_ = false;
   return CKT_OK;
 }
@@ -554,20 +565,31 @@ void Engine::drop_caches() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   graphs_.clear();
-  for (auto& kv : tile_cache_) (void)hipFree(kv.second.p);
+  for (auto& kv : tile_cache_) {
+    if (kv.second.pooled) (void)hipFreeAsync(kv.second.p, stream_);
+    else (void)hipFree(kv.second.p);
+  }
   tile_cache_.clear();
 }
 
+// Tile lists are allocated from the stream's memory pool and filled by a
+// copy from pinned staging queued on the stream, so building one for a new
+// set of circuits does not wait for the launches already queued.
 int Engine::upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out) {
   if (tile_cache_.size() >= 64) drop_caches();
   DevBuf b;
   b.n = n;
-  CKT_HIP(hipMalloc(&b.p, std::max<size_t>(bytes, 16)));
+  b.pooled = pool_ok_;
+  const size_t sz = std::max<size_t>(bytes, 16);
+  CKT_HIP(b.pooled ? hipMallocAsync(&b.p, sz, stream_) : hipMalloc(&b.p, sz));
   if (bytes) {
-    hipError_t e = hipMemcpy(b.p, data, bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(b.p);
-      return hip_fail(e, "hipMemcpy(tiles)");
+    void* staged = nullptr;
+    int st = stage(data, bytes, &staged);
+    hipError_t e = st ? hipSuccess : hipMemcpyAsync(b.p, staged, bytes, hipMemcpyHostToDevice, stream_);
+    if (st || e != hipSuccess) {
+      if (b.pooled) (void)hipFreeAsync(b.p, stream_);
+      else (void)hipFree(b.p);
+      return st ? st : hip_fail(e, "hipMemcpyAsync(tiles)");
     }
   }
   tile_cache_[key] = b;

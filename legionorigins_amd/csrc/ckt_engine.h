@@ -163,6 +163,7 @@ class Engine {
   struct DevBuf {
     void* p = nullptr;
     uint32_t n = 0;
+    bool pooled = false;  // from the stream's memory pool (freed stream-ordered)
   };
   struct Pending {
     int task;
