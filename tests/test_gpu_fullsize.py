@@ -161,3 +161,25 @@ def test_full_ten_loops_against_exact_and_fp64(C, O):
           f"{diff['fast+atomic']:.3e}, scaled+atomic {diff['scaled+atomic']:.3e}")
     assert max(diff.values()) <= TOL
     assert max(errs.values()) <= 1e-3
+
+
+@pytest.mark.parametrize("profile", [1, 0], ids=["stable", "reference"])
+def test_full_scaled_matches_exact_other_seed_pct50(C, profile):
+    # Another seed and half the wires crossing pieces (86% shared nodes):
+    # SCALED + fused ATOMIC against EXACT + CSR (bit-identical to the oracle)
+    # after 3 loops, every node and wire, by non-finite class and to 1e-5.
+    gp = C.GenParams(64, 16384, 65536, 50, 7, profile)
+    out = {}
+    for name, policy, mode in (("exact", C.POLICY_EXACT, C.REDUCE_CSR),
+                               ("scaled", C.POLICY_SCALED, C.REDUCE_ATOMIC)):
+        with C.Context(0) as ctx:
+            ctx.generate_device(gp)
+            ctx.run_loops(3, policy=policy, mode=mode)
+            out[name] = ctx.store()
+    (en, ew), (sn, sw) = out["exact"], out["scaled"]
+    assert_same_class(sn["voltage"], en["voltage"], "node voltage")
+    assert_same_class(sw["current"], ew["current"], "wire current")
+    assert_same_class(sw["voltage"], ew["voltage"], "wire voltage")
+    assert max_rel_err(sn["voltage"], en["voltage"]) <= TOL
+    assert max_wire_rel_err(sw["current"], ew["current"]) <= TOL
+    assert max_wire_rel_err(sw["voltage"], ew["voltage"]) <= TOL
