@@ -13,7 +13,7 @@ entry point, which takes wire records as they are):
 import numpy as np
 import pytest
 
-from helpers import TOL, assert_same_class, max_rel_err, max_wire_rel_err
+from helpers import TOL, assert_bit_identical, assert_same_class, max_rel_err, max_wire_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +110,30 @@ def test_scaled_matches_fast_on_random_parameters(C, O):
     assert (d < 2.0 ** -25).sum() > 100
     nonfin = ~np.isfinite(fast["current"]).all(axis=1)
     assert 0 < nonfin.sum() < W
+
+
+def test_degenerate_wire_parameters(C, O):
+    # Zero wire capacitance (the coupling d = dt/(R C) is infinite), zero
+    # inductance (no inductive term, g = 0), a huge resistance (d and g near
+    # zero) and ordinary wires, mixed in every wave.  Every policy gives the
+    # oracle's non-finite class on every element and its finite values to
+    # 1e-5 per wire; EXACT is bit-identical.
+    c = O.generate(2, 256, 1024, 95, 23, O.PROFILE_STABLE)
+    w = c.wires
+    k = np.arange(len(w)) % 4
+    w["capacitance"][k == 0] = np.float32(0)
+    w["inductance"][k == 1] = np.float32(0)
+    w["resistance"][k == 2] = np.float32(1e30)
+    ref = c.copy()
+    for p in c.pieces:
+        O.calc_new_currents(ref, p)
+    assert np.isnan(ref.wires["current"][k == 0]).all()
+    assert np.isfinite(ref.wires["current"][k != 0]).all()
+    for policy in (C.POLICY_EXACT, C.POLICY_RECIP, C.POLICY_FAST, C.POLICY_SCALED):
+        got = _run_accessor(C, c, policy)
+        for f in ("current", "voltage"):
+            if policy == C.POLICY_EXACT:
+                assert_bit_identical(got[f], ref.wires[f], f"EXACT {f}")
+                continue
+            assert_same_class(got[f], ref.wires[f], f"policy {policy} {f}")
+            assert max_wire_rel_err(got[f], ref.wires[f]) <= TOL, (policy, f)
