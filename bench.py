@@ -30,8 +30,8 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector/matrix peak (public spec)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (public spec)
 FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
 # FP32 flops each policy's step executes (its loop body's instruction mix):
-# FAST 20 FMA + 8 add; SCALED 10 FMA + 8 add, plus 10 multiplies per 4 steps.
-EXECUTED_FLOPS = {"fast": 48, "scaled": 30.5}
+# FAST 20 FMA + 8 add; SCALED 10 FMA + 6 add, plus 10 multiplies per 4 steps.
+EXECUTED_FLOPS = {"fast": 48, "scaled": 28.5}
 
 
 def log(msg):

@@ -272,16 +272,22 @@ struct Tridiag {
 // writing e = m^t w within a block of four steps leaves
 //   w_i <- c w_i + (w_{i-1} + w_{i+1}),   c = a / m,
 // one add and one accumulate-FMA per interior current and one FMA per end
-// current: 18 VALU per step, plus 10 multiplies by m^4 per block to bring
-// w back to e.  Every step updates all ten coupled currents; the currents
-// at any step are x* + m^t w, elementwise.
+// current, plus 10 multiplies by m^4 per block to bring w back to e.  The
+// step commutes with the mirror i -> 9 - i (A's two ends carry the same
+// coefficients), so the symmetric half s_i = w_i + w_{9-i} and the
+// antisymmetric half a_i = w_i - w_{9-i} (i < 5) each follow a five-point
+// stencil of the same form, closed at the middle by the mirror image
+// (w_5 = +-w_4 there): 8 VALU per half, 16 per step, against 18 on the ten
+// currents.  Every step updates all ten coupled values; the currents at any
+// step are x* + m^t (s +- a) / 2.
 //
 // The rewrite is used for a wire when one of two conditions shows that it
 // gives the reference's finite/non-finite outcome; other wires take the
 // plain tridiagonal steps:
 //   contracting: the step's infinity-norm |a_mid| + 2d (|a_end| + d at the
 //     ends) is at most 1/2, so e shrinks at least 2x per step and the
-//     scaled w stays below (|c|+2)^4 max|e|, checked to stay finite; m^4
+//     halves stay below 2 (|c|+2)^4 max|e| (every row of either half's
+//     stencil sums to at most |c|+2 in magnitude), checked to stay finite; m^4
 //     is a normal float.  Rounding of m^4 scales e, which has decayed below
 //     FP32 resolution of x* within a few dozen steps.
 //   diverging: the eigenvalues of A lie in [-g, -g + 4d] (A = a_mid I + m B
@@ -290,7 +296,8 @@ struct Tridiag {
 //     step, A is symmetric, so |e| >= 2^(steps) |e_0|, and if that passes
 //     FLT_MAX 16 steps before the end the reference's iteration has
 //     overflowed and spread NaN to every segment by its last step.  Then this
-//     one has overflowed too, and both end all-NaN.
+//     one has overflowed too (max(|s_i|, |a_i|) >= m^t |e_i| elementwise, and
+//     an infinite half stays non-finite), and both end all-NaN.
 // The reference parameters (g ~ 1e4..1e5) are diverging (and their inputs
 // non-finite after the first loop); the stable ones (g <= 0.1, d <= 1e-3)
 // contracting.
@@ -358,30 +365,53 @@ struct Relax<CKT_POLICY_SCALED> {
     float m4 = (t.m * t.m) * (t.m * t.m);
     asm volatile("" : "+v"(m4));
     // The deviation is formed again here, from opaque copies of x*, so that
-    // it is not kept live from the bound above across the fallback branch.
-    float e[kSeg];
+    // it is not kept live from the bound above across the fallback branch,
+    // and split into its mirror-symmetric and antisymmetric halves.
+    float s[kHalf], a[kHalf];
 #pragma unroll
-    for (int i = 0; i < kSeg; i++) {
-      asm volatile("" : "+v"(xs[i]));
-      e[i] = t.x[i] - xs[i];
+    for (int i = 0; i < kHalf; i++) {
+      asm volatile("" : "+v"(xs[i]), "+v"(xs[kSeg - 1 - i]));
+      const float lo = t.x[i] - xs[i], hi = t.x[kSeg - 1 - i] - xs[kSeg - 1 - i];
+      s[i] = lo + hi;
+      a[i] = lo - hi;
     }
     for (int j = 0; j < blocks; j++) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        float y[kSeg];
-        y[0] = fmaf(c_end, e[0], e[1]);
-#pragma unroll
-        for (int i = 1; i < kSeg - 1; i++) y[i] = fmaf(c_mid, e[i], e[i - 1] + e[i + 1]);
-        y[kSeg - 1] = fmaf(c_end, e[kSeg - 1], e[kSeg - 2]);
-#pragma unroll
-        for (int i = 0; i < kSeg; i++) e[i] = y[i];
+        half_step(s, c_end, c_mid, c_mid + 1.0f);
+        half_step(a, c_end, c_mid, c_mid - 1.0f);
       }
 #pragma unroll
-      for (int i = 0; i < kSeg; i++) e[i] *= m4;
+      for (int i = 0; i < kHalf; i++) {
+        s[i] *= m4;
+        a[i] *= m4;
+      }
     }
 #pragma unroll
-    for (int i = 0; i < kSeg; i++) t.x[i] = xs[i] + e[i];
+    for (int i = 0; i < kHalf; i++) {
+      t.x[i] = fmaf(0.5f, s[i] + a[i], xs[i]);
+      t.x[kSeg - 1 - i] = fmaf(0.5f, s[i] - a[i], xs[kSeg - 1 - i]);
+    }
     t.finish(I, V);
+  }
+
+ private:
+  static constexpr int kHalf = kSeg / 2;
+  static_assert(kSeg % 2 == 0 && kHalf >= 2, "the mirror split needs an even segment count");
+
+  // One step of one half: w_0 <- c_end w_0 + w_1 at the wire end,
+  // w_i <- c_mid w_i + (w_{i-1} + w_{i+1}) inside, and at the middle, where
+  // the mirror image w_{9-i} = +-w_i stands in for the missing neighbour,
+  // w_4 <- (c_mid +- 1) w_4 + w_3.
+  static __device__ __forceinline__ void half_step(float (&w)[kHalf], float c_end, float c_mid,
+                                                   float c_fold) {
+    float y[kHalf];
+    y[0] = fmaf(c_end, w[0], w[1]);
+#pragma unroll
+    for (int i = 1; i < kHalf - 1; i++) y[i] = fmaf(c_mid, w[i], w[i - 1] + w[i + 1]);
+    y[kHalf - 1] = fmaf(c_fold, w[kHalf - 1], w[kHalf - 2]);
+#pragma unroll
+    for (int i = 0; i < kHalf; i++) w[i] = y[i];
   }
 };
 
