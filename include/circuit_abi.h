@@ -117,9 +117,11 @@ enum {
  *           voltages are formed from the final currents.
  *   SCALED: FAST's update iterated as the deviation from its fixed point (one
  *           tridiagonal solve per wire and loop), scaled by the coupling
- *           coefficient so that each step is 8 adds and 10 FMAs, plus 10
- *           multiplies per 4 steps.  Used for wires shown to contract or to
- *           diverge; other wires take FAST's step.
+ *           coefficient and split into its mirror-symmetric and
+ *           antisymmetric halves (segment i against segment 9 - i), so that
+ *           each step is 6 adds and 10 FMAs, plus 10 multiplies per 4 steps.
+ *           Used for wires shown to contract or to diverge; other wires take
+ *           FAST's step.
  * RECIP, FAST and SCALED are not bit-exact; their error is checked against
  * 1e-5.  FAST and SCALED write a wire whose currents left the finite range as
  * all NaN, which is where the reference recurrence ends such a wire. */

@@ -279,10 +279,84 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
 
   std::vector<Ready> ready;
   std::vector<uint32_t> in_flight;  // requests advanced by the running iteration
-  // (circuit id, offset in `downloads`) of circuits finishing this iteration
-  std::vector<std::pair<uint32_t, size_t>> finishing;
-  PinnedBuffer downloads;
+  // Circuits finishing with an iteration, as (circuit id, offset in that
+  // iteration's download buffer).  Two buffers alternate: the voltages of the
+  // iteration that just ended are summed while the next one, launched first,
+  // copies its own finishers' voltages into the other buffer.
+  std::vector<std::pair<uint32_t, size_t>> finishing[2];
+  PinnedBuffer downloads[2];
+  int cur = 0;
   while (finished < n && !st) {
+    // Launch one iteration of every resident circuit with loops left.  It is
+    // the first thing after the previous iteration ended, so that the
+    // retirement, arrivals and admission below run while it does.
+    std::vector<PieceRef> refs;
+    in_flight.clear();
+    finishing[cur].clear();
+    for (uint32_t slot = 0; slot < E.slots().size(); slot++) {
+      Circuit* c = E.slots()[slot];
+      if (c->loops_left <= 0) continue;
+      in_flight.push_back(request_of_circuit[c->id]);
+      for (uint32_t k = 0; k < c->num_pieces; k++) refs.push_back(PieceRef{slot, k});
+    }
+    if (!refs.empty()) {
+      if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) break;
+      iterations++;
+      // Circuits that finish with this iteration: their final voltages are
+      // copied to pinned memory behind it, for the checksum after it ends.
+      size_t total = 0;
+      for (uint32_t ri : in_flight) {
+        const Circuit* c = E.find((uint32_t)circuit_of[ri]);
+        if (c->loops_left == 1) {
+          finishing[cur].emplace_back(c->id, total);
+          total += c->num_nodes;
+        }
+      }
+      if ((st = downloads[cur].reserve(total * sizeof(float)))) break;
+      for (auto& f : finishing[cur])
+        if ((st = E.copy_voltages_async(f.first, downloads[cur].at<float>(f.second)))) break;
+      if (st) break;
+    }
+    // Retire circuits with no loops left: the previous iteration's finishers,
+    // whose voltages are in the other buffer, and zero-loop requests.  None
+    // of them is in the iteration just launched.
+    std::vector<uint32_t> done;
+    for (Circuit* c : E.slots())
+      if (c->loops_left <= 0) done.push_back(c->id);
+    const auto& prev = finishing[cur ^ 1];
+    for (uint32_t id : done) {
+      const uint32_t ri = request_of_circuit[id];
+      double sum = 0;
+      auto f = std::find_if(prev.begin(), prev.end(),
+                            [id](const std::pair<uint32_t, size_t>& x) { return x.first == id; });
+      if (f != prev.end()) {
+        const float* v = downloads[cur ^ 1].at<float>(f->second);
+        for (uint32_t k = 0, nn = E.find(id)->num_nodes; k < nn; k++) sum += v[k];
+      } else {  // never ran (zero loops)
+        std::vector<float> v;
+        if ((st = E.download_voltages(id, v))) break;
+        for (float x : v) sum += x;
+      }
+      results[ri].final_voltage_checksum = sum;
+      if (s->keep_state) {
+        Circuit* c = E.find(id);
+        s->kept_nodes[ri].resize(c->num_nodes);
+        s->kept_wires[ri].resize(c->num_wires);
+        if ((st = E.store(id, s->kept_nodes[ri].data(), s->kept_wires[ri].data()))) break;
+      }
+    }
+    if (st) break;
+    // One removal for the whole set.  Slabs from the stream's memory pool are
+    // released in stream order, behind the running iteration, which does not
+    // use them; the slot table is rewritten by a copy queued behind it too.
+    if ((st = E.remove_many(done.data(), done.size()))) break;
+    for (uint32_t id : done) {
+      const uint32_t ri = request_of_circuit[id];
+      reserved -= need[ri];
+      admitted_wires -= wires_of[ri];
+      admitted--;
+      retire(ri, CKT_OK);
+    }
     const double now = clock();
     // Arrivals.
     while (next_arrival < order.size() && reqs[order[next_arrival]].arrival_s <= now) {
@@ -315,34 +389,6 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
       admitted_wires += wires_of[i];
       admitted++;
       pool.submit(i);
-    }
-    // Launch one iteration of every resident circuit with loops left.
-    std::vector<PieceRef> refs;
-    in_flight.clear();
-    for (uint32_t slot = 0; slot < E.slots().size(); slot++) {
-      Circuit* c = E.slots()[slot];
-      if (c->loops_left <= 0) continue;
-      in_flight.push_back(request_of_circuit[c->id]);
-      for (uint32_t k = 0; k < c->num_pieces; k++) refs.push_back(PieceRef{slot, k});
-    }
-    if (!refs.empty()) {
-      if ((st = E.loop_once(refs, cfg.policy, cfg.reduce_mode))) break;
-      iterations++;
-      // Circuits that finish with this iteration: their final voltages are
-      // copied to pinned memory behind it, for the checksum after the sync.
-      finishing.clear();
-      size_t total = 0;
-      for (uint32_t ri : in_flight) {
-        const Circuit* c = E.find((uint32_t)circuit_of[ri]);
-        if (c->loops_left == 1) {
-          finishing.emplace_back(c->id, total);
-          total += c->num_nodes;
-        }
-      }
-      if ((st = downloads.reserve(total * sizeof(float)))) break;
-      for (auto& f : finishing)
-        if ((st = E.copy_voltages_async(f.first, downloads.at<float>(f.second)))) break;
-      if (st) break;
     }
     // Overlap: enqueue the generation of planned circuits behind the
     // running iteration.
@@ -388,42 +434,7 @@ int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
         wire_updates += c->num_wires;
       }
     }
-    // Retire circuits with no loops left (including zero-loop requests).
-    std::vector<uint32_t> done;
-    for (Circuit* c : E.slots())
-      if (c->loops_left <= 0) done.push_back(c->id);
-    for (uint32_t id : done) {
-      const uint32_t ri = request_of_circuit[id];
-      double sum = 0;
-      auto f = std::find_if(finishing.begin(), finishing.end(),
-                            [id](const std::pair<uint32_t, size_t>& x) { return x.first == id; });
-      if (f != finishing.end()) {
-        const float* v = downloads.at<float>(f->second);
-        for (uint32_t k = 0, n = E.find(id)->num_nodes; k < n; k++) sum += v[k];
-      } else {  // never ran (zero loops)
-        std::vector<float> v;
-        if ((st = E.download_voltages(id, v))) break;
-        for (float x : v) sum += x;
-      }
-      results[ri].final_voltage_checksum = sum;
-      if (s->keep_state) {
-        Circuit* c = E.find(id);
-        s->kept_nodes[ri].resize(c->num_nodes);
-        s->kept_wires[ri].resize(c->num_wires);
-        if ((st = E.store(id, s->kept_nodes[ri].data(), s->kept_wires[ri].data()))) break;
-      }
-    }
-    if (st) break;
-    // One removal for the whole set; slabs from the stream's memory pool are
-    // released in stream order, without waiting.
-    if ((st = E.remove_many(done.data(), done.size()))) break;
-    for (uint32_t id : done) {
-      const uint32_t ri = request_of_circuit[id];
-      reserved -= need[ri];
-      admitted_wires -= wires_of[ri];
-      admitted--;
-      retire(ri, CKT_OK);
-    }
+    cur ^= 1;
   }
   const double elapsed = clock();
   // Leave nothing resident, whatever happened.
