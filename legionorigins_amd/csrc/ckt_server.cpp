@@ -17,6 +17,8 @@
 //   retire    -> a circuit whose loops are done is checksummed (its final
 //                voltages are copied behind its last iteration), optionally
 //                kept, and its slab released.
+// Each pass launches the next iteration first and then retires, takes
+// arrivals and admits while it runs, so the GPU waits only for the launch.
 // Circuits are independent, so a circuit's result does not depend on what
 // else shares its iterations (bit-identical with EXACT + CSR).
 #include <time.h>

@@ -60,6 +60,28 @@ def test_served_results_equal_alone(C, policy, mode):
     s.close()
 
 
+def test_served_checksums_with_zero_loop_requests(C):
+    # Without kept state, the path the benchmark serves: finishers are summed
+    # from the pinned buffer of their last iteration while the next one runs;
+    # zero-loop requests finish without running, on their generated state.
+    rng = np.random.default_rng(11)
+    reqs, t = [], 0.0
+    for i in range(12):
+        t += rng.exponential(0.005)
+        W = int(np.exp(rng.uniform(np.log(2**10), np.log(2**14))))
+        loops = 0 if i % 4 == 1 else int(rng.integers(1, 6))
+        reqs.append(C.Request(C.shape_for_wires(W, 95, 700 + i, C.PROFILE_STABLE), loops, t))
+    s = C.Server(0, max_active=5, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR, gen_threads=3)
+    results, rep = s.run_trace(reqs)
+    assert rep["completed"] == len(reqs) and rep["rejected"] == 0 and rep["iterations"] > 0
+    for r, res in zip(reqs, results):
+        assert res["status"] == 0
+        n1, _ = alone(C, r.gen, r.num_loops, C.POLICY_EXACT, C.REDUCE_CSR)
+        v = n1["voltage"].astype(np.float64)
+        assert abs(res["checksum"] - v.sum()) <= 1e-12 * np.abs(v).sum(), (r.num_loops, res)
+    s.close()
+
+
 def test_server_sheds_requests_past_their_admission_deadline(C):
     # One slot: the first request occupies it for 20 loops; the others, all
     # arriving at t=0 with a 1 ms admission deadline, time out in the queue.
