@@ -29,9 +29,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector/matrix peak (public spec)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (public spec)
 FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
-# FP32 flops each policy's step executes (its loop body's instruction mix):
-# FAST 20 FMA + 8 add; SCALED 10 FMA + 6 add, plus 10 multiplies per 4 steps.
-EXECUTED_FLOPS = {"fast": 48, "scaled": 28.5}
+# FP32 flops each policy's step executes (an FMA counts 2), from its loop
+# body's instruction mix, which the SQ_INSTS_VALU_FLOPS_FP32 counter confirms
+# per launch (profiles/calc_new_currents_4M_wires_*_counters.md):
+#   EXACT  the reference's 67 FP32 operations; its 29 divisions are FP64
+#          multiplies by a per-wire reciprocal (29 FP64 flops, not counted here)
+#   RECIP  10 x (subtract + 2 FMA) + 9 x (subtract + FMA) = 77
+#   FAST   20 FMA + 8 add = 48
+#   SCALED 10 FMA + 6 add, plus 10 multiplies per 4 steps = 28.5
+EXECUTED_FLOPS = {"exact": 67, "recip": 77, "fast": 48, "scaled": 28.5}
 
 
 def log(msg):
@@ -498,9 +504,11 @@ def main():
     rep, wall, k = time_loops(C, ctx, args.steps, policy, mode)
     value = W * args.steps / wall
     cnc_ms, cnc_n = k[C.CALC_NEW_CURRENTS]
-    achieved = FLOPS_PER_WIRE_STEP * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
+    ef = EXECUTED_FLOPS[args.policy]
+    achieved = ef * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
+    ref_equiv = FLOPS_PER_WIRE_STEP * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
     log(f"[bench] {value:.4g} wire-updates/s; calc_new_currents {cnc_ms:.3f} ms/launch "
-        f"({achieved:.1f} TFLOP/s algorithmic)")
+        f"({achieved:.1f} TFLOP/s executed, {ref_equiv:.1f} reference-equivalent)")
     out = {
         "metric": "wire-updates/s at 1 GPU (calc_new_currents + distribute_charge + "
                   "update_voltages per loop)",
@@ -520,21 +528,22 @@ def main():
                                f"WIRE_SEGMENTS=10, 10000 inner steps/loop, FP32, SoA",
                    "policy": args.policy, "reduction": args.mode,
                    "parallelism": "single GPU"},
+        # calc_new_currents is FP32 vector-ALU bound.  `achieved` is the FP32
+        # work the kernel executes (EXECUTED_FLOPS per wire-step x 10000 steps
+        # x wires) over its mean launch time from HIP events on the launch
+        # stream; `reference_equivalent` is the reference's own count (96 per
+        # wire-step, circuit.cc:320-334) over the same time, which exceeds the
+        # peak for policies that reach the same values with fewer operations.
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "calc_new_currents",
-                     "flops_per_launch": FLOPS_PER_WIRE_STEP * C.STEPS * W,
-                     "avg_launch_ms": cnc_ms, "launches": cnc_n},
+                     "executed_flops_per_wire_step": ef,
+                     "flops_per_launch": ef * C.STEPS * W,
+                     "avg_launch_ms": cnc_ms, "launches": cnc_n,
+                     "reference_equivalent": {
+                         "flops_per_wire_step": FLOPS_PER_WIRE_STEP, "tflops": ref_equiv,
+                         "ratio_to_peak": ref_equiv / PEAK_FP32_TFLOPS}},
     }
-    if args.policy in EXECUTED_FLOPS:
-        # FP32 work the step executes per wire-step (the kernel's instruction
-        # mix; the VALU counters agree), against the algorithmic 96 of the
-        # reference formula above.
-        ef = EXECUTED_FLOPS[args.policy]
-        executed = ef * C.STEPS * W / (cnc_ms * 1e-3) / 1e12
-        out["roofline"]["executed"] = {
-            "flops_per_wire_step": ef, "tflops": executed,
-            "frac": executed / PEAK_FP32_TFLOPS}
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
@@ -542,6 +551,9 @@ def main():
         key = f"calc_new_currents/{args.workload}/{args.policy}"
         if key in t:
             out["roofline"]["traffic"] = t[key]["bytes_per_launch"]
+            out["roofline"]["traffic_source"] = ("profiles/traffic.json: rocprofv3 --pmc "
+                                                 "FETCH_SIZE and WRITE_SIZE of this launch, "
+                                                 "scaled by a plain copy's counters")
     detail = {"kernels_ms": {"calc_new_currents": cnc_ms,
                              "distribute_charge": k[C.DISTRIBUTE_CHARGE][0],
                              "update_voltages": k[C.UPDATE_VOLTAGES][0]},
@@ -553,8 +565,12 @@ def main():
             loops = 2 if pv == C.POLICY_EXACT else 4
             r2, w2, k2 = time_loops(C, ctx, loops, pv, mode)
             ms = k2[C.CALC_NEW_CURRENTS][0]
+            ex = EXECUTED_FLOPS[name] * C.STEPS * W / ms / 1e9
             pol[name] = {"wire_updates_per_s": W * loops / w2, "cnc_ms": ms,
-                         "tflops_algorithmic": FLOPS_PER_WIRE_STEP * C.STEPS * W / ms / 1e9}
+                         "executed_flops_per_wire_step": EXECUTED_FLOPS[name],
+                         "tflops_executed": ex, "frac_of_fp32_peak": ex / PEAK_FP32_TFLOPS,
+                         "tflops_reference_equivalent":
+                             FLOPS_PER_WIRE_STEP * C.STEPS * W / ms / 1e9}
             log(f"[bench] policy {name}: {pol[name]}")
         detail["policies"] = pol
         red = {}
@@ -596,7 +612,8 @@ def main():
         ms = k5[C.CALC_NEW_CURRENTS][0]
         detail["stable_profile"] = {
             "wire_updates_per_s": W * 4 / w5, "cnc_ms": ms,
-            "tflops_algorithmic": FLOPS_PER_WIRE_STEP * C.STEPS * W / ms / 1e9}
+            "tflops_executed": ef * C.STEPS * W / ms / 1e9,
+            "tflops_reference_equivalent": FLOPS_PER_WIRE_STEP * C.STEPS * W / ms / 1e9}
         log(f"[bench] stable profile: {detail['stable_profile']}")
         ctx_s.close()
     out["detail"] = detail

@@ -47,15 +47,36 @@ __device__ __forceinline__ float div_fixed(float a, double rb, float b) {
   return q;
 }
 
+// The reference's step in its order of operations (circuit_gpu.cu:114-129),
+// `steps` times, on the running currents ni and voltages nv (nv[0], nv[kSeg]
+// the node voltages), with I0 and V0 the wire's values at the start of the
+// loop; writes the result to I0 and V0.  Every division is correctly rounded
+// (div_fixed).
+__device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[kSeg + 1],
+                                                float (&I0)[kSeg], float (&V0)[kSeg - 1], float L,
+                                                float R, float C, int steps) {
+  const float dt = CKT_DT_F;
+  constexpr double rdt = 1.0 / (double)CKT_DT_F;
+  const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
+  for (int j = 0; j < steps; j++) {
+#pragma unroll
+    for (int i = 0; i < kSeg; i++)
+      ni[i] = div_fixed((nv[i + 1] - nv[i]) - div_fixed(L * (ni[i] - I0[i]), rdt, dt), rR, R);
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++)
+      nv[i + 1] = V0[i] + div_fixed(dt * (ni[i] - ni[i + 1]), rC, C);
+  }
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) I0[i] = ni[i];
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) V0[i] = nv[i + 1];
+}
+
 // Reference order of operations; bit-identical to a strict FP32 CPU run.
-// Every division is correctly rounded (div_fixed).
 template <>
 struct Relax<CKT_POLICY_EXACT> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
-    const float dt = CKT_DT_F;
-    constexpr double rdt = 1.0 / (double)CKT_DT_F;
-    const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
     float ni[kSeg], nv[kSeg + 1];
 #pragma unroll
     for (int i = 0; i < kSeg; i++) ni[i] = I[i];
@@ -63,20 +84,125 @@ struct Relax<CKT_POLICY_EXACT> {
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
     nv[kSeg] = vout;
-    for (int j = 0; j < steps; j++) {
-#pragma unroll
-      for (int i = 0; i < kSeg; i++)
-        ni[i] = div_fixed((nv[i + 1] - nv[i]) - div_fixed(L * (ni[i] - I[i]), rdt, dt), rR, R);
-#pragma unroll
-      for (int i = 0; i < kSeg - 1; i++)
-        nv[i + 1] = V[i] + div_fixed(dt * (ni[i] - ni[i + 1]), rC, C);
-    }
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) I[i] = ni[i];
-#pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) V[i] = nv[i + 1];
+    reference_steps(ni, nv, I, V, L, R, C, steps);
   }
 };
+
+// The other policies run the last kTail steps of a loop in the reference's
+// order.  Where a wire's values leave the finite range, the reference
+// recurrence turns them to NaN one segment per step (an infinite current
+// makes the voltage difference across its segment infinite, and the
+// inductive term L*(I - I0)/dt the same infinity, so the next step takes
+// inf - inf); which of a wire's values are +-inf, NaN or still finite at the
+// end depends on the exact step at which each value overflowed, and that
+// step depends on the reference's intermediate values (L*(I - I0)/dt is R
+// times the current it yields).  A wire whose values are still finite kTail
+// steps before the end therefore finishes in the reference's order and
+// overflows where the reference does.  A wire that is already non-finite
+// then overflowed within a step of the reference, whose NaN then reaches
+// every current and internal voltage in at most 10 further steps (1.5 M
+// wires with NaN, infinite or near-FLT_MAX node voltages in both generator
+// profiles needed 10 at most), so it ends all-NaN, as it is written.
+constexpr int kTail = 16;
+
+// Wires with a mode whose gain per step is moderate in magnitude also run
+// the whole loop in the reference's order.  Each step's rounding perturbs the
+// state by a relative ~1e-7, and a mode with gain lambda carries the
+// perturbation for about 1/(1 - |lambda|) steps, or, for |lambda| > 1, keeps
+// amplifying it with the state: the reassociated steps then drift from the
+// reference's by ~1e-7 per step the mode has grown through (1e-3 after 10000
+// steps near |lambda| = 1, 1.4e-5 after 120 steps at |lambda| = 2.2), and
+// where such a wire overflows near the end of the loop the drift moves the
+// step it overflows at.  The step's modes are those of the tridiagonal update
+// x <- K + A x with A = (2d - g) I - d B, B the path adjacency plus ones at
+// both ends, whose eigenvalues are 2 cos(pi k / 10):
+//   lambda_k = 2d - g - 2d cos(pi k / 10),  g = L / (dt R),  d = dt / (R C).
+// Outside 0.95 <= |lambda_k| <= 1000 a perturbation decays by 5% per step
+// (the accumulated drift stays below ~20 roundings) or grows over 1000x per
+// step, so that a wire starting from values of at least 2^-96 (see
+// tiny_input) overflows within 23 steps (1000^23 > FLT_MAX / 2^-96), within a
+// step of the reference.  No wire of either generator profile comes near the
+// band: the reference parameters have |lambda| ~ 1e4..1e5, the stable ones
+// |lambda| < 0.12.
+struct ModeGains {
+  bool moderate;  // some |lambda_k| in [0.95, 1000]
+  bool fast;      // some |lambda_k| above 1000
+};
+
+__device__ __forceinline__ ModeGains mode_gains(float L, float R, float C) {
+  static_assert(kSeg == 10, "the table holds cos(pi k / 10)");
+  constexpr float kCos[kSeg] = {1.0f,         0.95105652f,  0.80901699f,  0.58778525f,
+                                0.30901699f,  0.0f,         -0.30901699f, -0.58778525f,
+                                -0.80901699f, -0.95105652f};
+  const float dt = CKT_DT_F;
+  const float g = (L / dt) / R, d = (1.0f / R) * (dt / C);
+  const float a = 2.0f * d - g, m2 = -2.0f * d;
+  ModeGains mg{false, false};
+#pragma unroll
+  for (int k = 0; k < kSeg; k++) {
+    const float lam = fabsf(fmaf(m2, kCos[k], a));
+    mg.moderate |= lam >= 0.95f && lam <= 1000.0f;
+    mg.fast |= lam > 1000.0f;
+  }
+  return mg;
+}
+
+// A wire with a fast-growing mode also runs in the reference's order when a
+// value it starts from is nonzero but below 2^-96 in magnitude.  The
+// reference's products L*(I - I0) and dt*(I_i - I_i+1) of such values fall
+// below the normal range and lose bits or become 0 (a growing mode started
+// from them then stays 0), where the reassociated steps scale them by other
+// factors.  In a wire whose modes all decay, such values are rounding-level
+// parts of a state its fixed point dominates, and they take the policy's own
+// steps (values of 1e-30 occur among the internal voltages of the stable
+// profile's early loops).
+__device__ __forceinline__ bool tiny_input(const float (&I)[kSeg], const float (&V)[kSeg - 1],
+                                           float vin, float vout) {
+  bool tiny = (vin != 0.0f && fabsf(vin) < 0x1p-96f) || (vout != 0.0f && fabsf(vout) < 0x1p-96f);
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) tiny |= I[i] != 0.0f && fabsf(I[i]) < 0x1p-96f;
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) tiny |= V[i] != 0.0f && fabsf(V[i]) < 0x1p-96f;
+  return tiny;
+}
+
+// Whether a wire runs its whole loop in the reference's order (the cases
+// above).
+__device__ __forceinline__ bool whole_loop_in_reference_order(const float (&I)[kSeg],
+                                                              const float (&V)[kSeg - 1],
+                                                              float vin, float vout, float L,
+                                                              float R, float C, int steps) {
+  if (steps <= kTail) return true;
+  const ModeGains mg = mode_gains(L, R, C);
+  return mg.moderate || (mg.fast && tiny_input(I, V, vin, vout));
+}
+
+// The tail: the internal voltages formed from the body's currents x by the
+// reference's voltage update, then the reference's steps.  A non-finite x
+// gives an all-NaN wire.
+__device__ __forceinline__ void finish_in_reference_order(float (&I)[kSeg], float (&V)[kSeg - 1],
+                                                          float (&x)[kSeg], float vin, float vout,
+                                                          float L, float R, float C) {
+  bool finite = true;
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) finite &= __builtin_isfinite(x[i]);
+  if (!finite) {
+    const float nan = __builtin_nanf("");
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) I[i] = nan;
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) V[i] = nan;
+    return;
+  }
+  const float dt = CKT_DT_F;
+  const double rC = 1.0 / (double)C;
+  float nv[kSeg + 1];
+  nv[0] = vin;
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i] + div_fixed(dt * (x[i] - x[i + 1]), rC, C);
+  nv[kSeg] = vout;
+  reference_steps(x, nv, I, V, L, R, C, kTail);
+}
 
 // Per-wire constants hoisted, FMAs in the loop, voltages kept every step:
 //   I_i <- h*(v_{i+1}-v_i) - g*(I_i - I0_i),  h = 1/R, g = (L/dt)/R
@@ -85,6 +211,10 @@ template <>
 struct Relax<CKT_POLICY_RECIP> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
+    if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
+      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
+      return;
+    }
     const float dt = CKT_DT_F;
     const float h = 1.0f / R, g = (L / dt) / R, b = dt / C;
     float c0[kSeg], ni[kSeg], nv[kSeg + 1];
@@ -97,16 +227,22 @@ struct Relax<CKT_POLICY_RECIP> {
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
     nv[kSeg] = vout;
-    for (int j = 0; j < steps; j++) {
+    // Step 1 from the seeded voltages: the running currents still equal I0,
+    // so the inductive term is zero, as the reference computes it (its
+    // L*(I - I0) is exactly 0 there; -g*I0 + g*I0 would leave the rounding of
+    // g*I0, which steps with a large gain g amplify).
+#pragma unroll
+    for (int i = 0; i < kSeg; i++) ni[i] = h * (nv[i + 1] - nv[i]);
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = fmaf(b, ni[i] - ni[i + 1], V[i]);
+#pragma unroll 2
+    for (int j = kTail + 1; j < steps; j++) {
 #pragma unroll
       for (int i = 0; i < kSeg; i++) ni[i] = fmaf(-g, ni[i], fmaf(h, nv[i + 1] - nv[i], c0[i]));
 #pragma unroll
       for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = fmaf(b, ni[i] - ni[i + 1], V[i]);
     }
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) I[i] = ni[i];
-#pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) V[i] = nv[i + 1];
+    finish_in_reference_order(I, V, ni, vin, vout, L, R, C);
   }
 };
 
@@ -119,12 +255,17 @@ struct Relax<CKT_POLICY_RECIP> {
 // because the running current still equals I0).  28 VALU per step: for the
 // eight interior currents one add (I_{i-1} + I_{i+1}, two register operands)
 // and two FMAs, for the two end currents two FMAs.  The add form issues
-// faster than a third three-operand FMA.
+// faster than a third three-operand FMA.  The last kTail steps run in the
+// reference's order.
 template <>
 struct Relax<CKT_POLICY_FAST> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
-    if (steps <= 0) return;
+    if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
+      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
+      return;
+    }
+    const int body = steps - kTail;
     const float dt = CKT_DT_F;
     const float h = 1.0f / R, g = (L / dt) / R, b = dt / C;
     const float d = h * b;
@@ -157,35 +298,16 @@ struct Relax<CKT_POLICY_FAST> {
     _Pragma("unroll") for (int i = 0; i < kSeg; i++) x[i] = y[i];                  \
   }
     int j = 1;
-    for (; j + 4 <= steps; j += 4) {
+    for (; j + 4 <= body; j += 4) {
       CKT_FAST_STEP CKT_FAST_STEP CKT_FAST_STEP CKT_FAST_STEP
     }
-    for (; j < steps; j++) CKT_FAST_STEP
+    for (; j < body; j++) CKT_FAST_STEP
 #undef CKT_FAST_STEP
-#pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) V[i] = fmaf(b, x[i] - x[i + 1], V[i]);
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) I[i] = x[i];
-    // Non-finite states.  In the reference recurrence a current that turns
-    // +-inf is NaN one step later: the voltage difference across its segment
-    // is then +-inf or NaN with the current's sign, and the inductive term
-    // L*(I - I0)/dt is the same infinity.  NaN then spreads one segment per
-    // step through the voltages, so a wire that leaves the finite range ends
-    // with every current and internal voltage NaN (the reference parameters
-    // do this within ~10 of the 10000 steps).  The substitution above has no
-    // voltages in its step, so same-signed infinite currents would stay
-    // infinite; non-finite values never become finite again in it, so the
-    // end state says whether the wire left the finite range.
-    bool finite = true;
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) finite &= __builtin_isfinite(x[i]);
-    if (!finite) {
-      const float nan = __builtin_nanf("");
-#pragma unroll
-      for (int i = 0; i < kSeg; i++) I[i] = nan;
-#pragma unroll
-      for (int i = 0; i < kSeg - 1; i++) V[i] = nan;
-    }
+    // The substitution has no voltages in its step, so same-signed infinite
+    // currents would stay infinite where the reference's turn to NaN; values
+    // never become finite again in it, so a non-finite x here means the wire
+    // left the finite range (finish_in_reference_order writes it all-NaN).
+    finish_in_reference_order(I, V, x, vin, vout, L, R, C);
   }
 };
 
@@ -244,26 +366,6 @@ struct Tridiag {
       step();
     }
   }
-
-  // Writes the currents and the internal voltages formed from them, a wire
-  // with a non-finite current as all NaN (as FAST does, for the reason given
-  // there).
-  __device__ __forceinline__ void finish(float (&I)[kSeg], float (&V)[kSeg - 1]) const {
-#pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) V[i] = fmaf(b, x[i] - x[i + 1], V[i]);
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) I[i] = x[i];
-    bool finite = true;
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) finite &= __builtin_isfinite(x[i]);
-    if (!finite) {
-      const float nan = __builtin_nanf("");
-#pragma unroll
-      for (int i = 0; i < kSeg; i++) I[i] = nan;
-#pragma unroll
-      for (int i = 0; i < kSeg - 1; i++) V[i] = nan;
-    }
-  }
 };
 
 // The tridiagonal update x <- K + A x, iterated as the deviation from its
@@ -294,10 +396,12 @@ struct Tridiag {
 //     with B the path adjacency plus ones at both ends, eigenvalues
 //     2 cos(pi k / 10)); if -g + 4d <= -2 every mode grows at least 2x per
 //     step, A is symmetric, so |e| >= 2^(steps) |e_0|, and if that passes
-//     FLT_MAX 16 steps before the end the reference's iteration has
-//     overflowed and spread NaN to every segment by its last step.  Then this
-//     one has overflowed too (max(|s_i|, |a_i|) >= m^t |e_i| elementwise, and
-//     an infinite half stays non-finite), and both end all-NaN.
+//     FLT_MAX by the end of the blocks, kTail steps before the end, the
+//     reference's iteration has overflowed and spread NaN to every segment
+//     by its last step.  Then this one has overflowed too (max(|s_i|, |a_i|)
+//     >= m^t |e_i| elementwise, and an infinite half stays non-finite), and
+//     both end all-NaN.
+// Either way the last kTail steps run in the reference's order.
 // The reference parameters (g ~ 1e4..1e5) are diverging (and their inputs
 // non-finite after the first loop); the stable ones (g <= 0.1, d <= 1e-3)
 // contracting.
@@ -305,15 +409,25 @@ template <>
 struct Relax<CKT_POLICY_SCALED> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
-    if (steps <= 0) return;
-    Tridiag t(I, V, vin, vout, L, R, C);
-    const int rest = steps - 1;
-    for (int j = 0; j < (rest & 3); j++) t.step();
-    const int blocks = rest >> 2;
-    if (blocks == 0) {
-      t.finish(I, V);
+    if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
+      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
       return;
     }
+    Tridiag t(I, V, vin, vout, L, R, C);
+    const int rest = steps - kTail - 1;
+    for (int j = 0; j < (rest & 3); j++) t.step();
+    const int blocks = rest >> 2;
+    if (blocks > 0) run_blocks(t, blocks);
+    finish_in_reference_order(I, V, t.x, vin, vout, L, R, C);
+  }
+
+ private:
+  static constexpr int kHalf = kSeg / 2;
+  static_assert(kSeg % 2 == 0 && kHalf >= 2, "the mirror split needs an even segment count");
+
+  // `blocks` blocks of four steps from t.x, left in t.x: rewritten where one
+  // of the two conditions holds, as plain tridiagonal steps otherwise.
+  static __device__ __forceinline__ void run_blocks(Tridiag& t, int blocks) {
     // x*: (1 - a_i) x*_i + d (x*_{i-1} + x*_{i+1}) = K_i, by the Thomas
     // algorithm (diagonally dominant under either condition below).
     float xs[kSeg];
@@ -346,9 +460,10 @@ struct Relax<CKT_POLICY_SCALED> {
     const float grow = norm / t.d;
     const bool contracting =
         norm <= 0.5f && (!e_finite || emax * ((grow * grow) * (grow * grow)) <= 0x1p120f);
-    // Steps of guaranteed growth, leaving 16 at the end for the reference's
-    // NaN to reach every segment; 2^132 max|e| is past FLT_MAX * sqrt(10).
-    const int grown = min(4 * blocks - 16, 256);
+    // Steps of guaranteed growth: the blocks, which end kTail steps before
+    // the end, time enough for the reference's NaN to reach every segment;
+    // 2^132 max|e| is past FLT_MAX * sqrt(10).
+    const int grown = min(4 * blocks, 256);
     const bool diverging = t.g - 4.0f * t.d >= 2.0f && grown > 0 &&
                            (!e_finite || (emax > 0.0f && emax >= ldexpf(1.0f, 132 - grown)));
     // d >= 2^-25 keeps m^4 a normal float (>= 2^-100).  Under either
@@ -358,7 +473,6 @@ struct Relax<CKT_POLICY_SCALED> {
     // then, and so does either step here, the deviation staying non-finite.
     if (!(t.d >= 0x1p-25f && (contracting || diverging))) {
       t.steps4(blocks);
-      t.finish(I, V);
       return;
     }
     const float c_mid = t.a_mid / t.m, c_end = t.a_end / t.m;
@@ -392,12 +506,7 @@ struct Relax<CKT_POLICY_SCALED> {
       t.x[i] = fmaf(0.5f, s[i] + a[i], xs[i]);
       t.x[kSeg - 1 - i] = fmaf(0.5f, s[i] - a[i], xs[kSeg - 1 - i]);
     }
-    t.finish(I, V);
   }
-
- private:
-  static constexpr int kHalf = kSeg / 2;
-  static_assert(kSeg % 2 == 0 && kHalf >= 2, "the mirror split needs an even segment count");
 
   // One step of one half: w_0 <- c_end w_0 + w_1 at the wire end,
   // w_i <- c_mid w_i + (w_{i-1} + w_{i+1}) inside, and at the middle, where

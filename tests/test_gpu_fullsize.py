@@ -12,7 +12,10 @@ through size-independent properties.
     the oracle's NaN pattern on the sample;
   * the benchmark's configuration (FAST, fused ATOMIC) within 1e-5 of the
     oracle on a sample of wires and on every node voltage, and with the
-    oracle's non-finite classes under the reference parameters.
+    oracle's non-finite classes under the reference parameters;
+  * all 10 loops of the workload, loop by loop, for every policy in both
+    profiles: each GPU loop against the oracle's task bodies run from the
+    GPU's state before it, on a sample of wires and on every node.
 """
 import numpy as np
 import pytest
@@ -73,7 +76,7 @@ def test_full_atomic_and_csr_agree(C):
     assert max_rel_err(out[C.REDUCE_ATOMIC]["voltage"], out[C.REDUCE_CSR]["voltage"]) <= TOL
 
 
-@pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
+@pytest.mark.parametrize("policy", [1, 2, 3], ids=["recip", "fast", "scaled"])
 def test_full_fast_sampled_wires_within_tolerance(C, O, policy):
     # The benchmark's configuration (FAST, fused ATOMIC reduction) at full
     # size, stable profile: one loop, a random sample of wires against the
@@ -104,7 +107,7 @@ def test_full_fast_sampled_wires_within_tolerance(C, O, policy):
     assert max_rel_err(nodes["voltage"], want) <= TOL
 
 
-@pytest.mark.parametrize("policy", [2, 3], ids=["fast", "scaled"])
+@pytest.mark.parametrize("policy", [1, 2, 3], ids=["recip", "fast", "scaled"])
 def test_full_fast_reference_profile_classes_after_one_loop(C, O, policy):
     # The benchmark's configuration and profile (reference parameters, FAST,
     # fused ATOMIC), one loop: every sampled wire's currents and internal
@@ -183,3 +186,83 @@ def test_full_scaled_matches_exact_other_seed_pct50(C, profile):
     assert max_rel_err(sn["voltage"], en["voltage"]) <= TOL
     assert max_wire_rel_err(sw["current"], ew["current"]) <= TOL
     assert max_wire_rel_err(sw["voltage"], ew["voltage"]) <= TOL
+
+
+def _node_update_f64(before_nodes, wires):
+    """distribute_charge + update_voltages (circuit.cc:415-460) in float64
+    from the wires' currents onto the nodes' state before the loop."""
+    dt = np.float32(1e-6)
+    N = len(before_nodes)
+    q = np.bincount(wires["in_ptr"], (-dt * wires["current"][:, 0]).astype(np.float64), N)
+    q += np.bincount(wires["out_ptr"], (dt * wires["current"][:, 9]).astype(np.float64), N)
+    q += before_nodes["charge"].astype(np.float64)
+    v0 = before_nodes["voltage"].astype(np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        return ((v0 + q / before_nodes["capacitance"]) *
+                (1.0 - before_nodes["leakage"].astype(np.float64)))
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2, 3], ids=["exact+csr", "recip", "fast", "scaled"])
+@pytest.mark.parametrize("profile", [1, 0], ids=["stable", "reference"])
+def test_full_ten_loops_loop_by_loop_against_oracle(C, O, profile, policy):
+    # The full-size workload for its 10 loops.  Before each GPU loop the
+    # state is read back; the oracle's task bodies then run that loop from
+    # it: calc_new_currents (circuit.cc:352-413) on 1500 sampled wires, and
+    # distribute_charge + update_voltages (circuit.cc:415-460) over every
+    # node from the GPU's new currents.  EXACT with the CSR reduction is
+    # compared bit for bit (wires and every node); RECIP, FAST and SCALED with
+    # the fused ATOMIC reduction by non-finite class on every compared value
+    # and within 1e-5 where finite (the node update against a float64 sum of
+    # the contributions).
+    P, npp, wpp, pct, seed = FULL
+    gp = C.GenParams(P, npp, wpp, pct, seed, profile)
+    mode = C.REDUCE_CSR if policy == C.POLICY_EXACT else C.REDUCE_ATOMIC
+    rng = np.random.default_rng(10 + policy)
+    threads = 16
+    worst = {"wire": 0.0, "node": 0.0}
+    with C.Context(0) as ctx:
+        ctx.generate_device(gp)
+        before_n, before_w = ctx.store()
+        sample = np.sort(rng.choice(len(before_w), 1500, replace=False)).astype(np.uint32)
+        pieces = np.zeros(P, O.PIECE)
+        pieces["first_wire"] = np.arange(P) * wpp
+        pieces["num_wires"] = wpp
+        pieces["first_node"] = np.arange(P) * npp
+        pieces["num_nodes"] = npp
+        private = np.ones(P * npp, np.int32)  # one node array serves every instance
+        for loop in range(10):
+            ctx.run_loops(1, policy=policy, mode=mode)
+            after_n, after_w = ctx.store()
+            # calc_new_currents on the sample, from the state before the loop
+            w = before_w.copy()
+            O.lib().orc_cnc_sample_f32(0, O.STEPS, threads, O._p(w), O._p(before_n),
+                                       O._p(sample), len(sample))
+            for f in ("current", "voltage"):
+                if policy == C.POLICY_EXACT:
+                    assert_bit_identical(after_w[f][sample], w[f][sample], f"loop {loop} {f}")
+                    continue
+                assert_same_class(after_w[f][sample], w[f][sample], f"loop {loop} {f}")
+                fin = np.isfinite(w[f][sample]).all(axis=1)
+                e = max_wire_rel_err(after_w[f][sample][fin], w[f][sample][fin])
+                worst["wire"] = max(worst["wire"], e)
+                assert e <= TOL, (loop, f, e)
+            # distribute_charge + update_voltages over every node
+            if policy == C.POLICY_EXACT:
+                ref = O.Circuit(before_n.copy(), after_w, private, pieces)
+                for p in pieces:
+                    O.distribute_charge(ref, p)
+                for p in pieces:
+                    O.update_voltages(ref, p)
+                assert_bit_identical(after_n["voltage"], ref.nodes["voltage"], f"loop {loop} nodes")
+            else:
+                want = _node_update_f64(before_n, after_w)
+                assert_same_class(after_n["voltage"], want.astype(np.float32),
+                                  f"loop {loop} node voltage class")
+                e = max_rel_err(after_n["voltage"], want)
+                worst["node"] = max(worst["node"], e)
+                assert e <= TOL, (loop, e)
+            assert not after_n["charge"].any()
+            before_n, before_w = after_n, after_w
+    print(f"\n10 loops, profile {profile}, policy {policy}: worst sampled-wire error "
+          f"{worst['wire']:.2e}, worst node-voltage error {worst['node']:.2e}; "
+          f"NaN node voltages at the end: {np.isnan(before_n['voltage']).mean():.4f}")

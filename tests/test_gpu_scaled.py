@@ -6,7 +6,8 @@ Wires are given parameters from four regimes (through the accessor-form
 entry point, which takes wire records as they are):
   * contracting (the stable profile): the scaled step, within 1e-5 of the
     oracle, and not bit-equal to FAST (so the scaled path really ran);
-  * marginal (L/(dt R) = 1, an undamped mode): FAST's step, bit-equal to FAST;
+  * marginal (L/(dt R) = 1, an undamped mode): the whole loop in the
+    reference's order, as FAST runs it, bit-equal to FAST;
   * tiny coupling (wire capacitance 1e4, d < 2^-25): FAST's step, bit-equal;
   * diverging (the reference inductance): all-NaN, as in the oracle.
 """
@@ -85,7 +86,9 @@ def test_scaled_matches_fast_on_random_parameters(C, O):
     # contracting, diverging, marginal and tiny-coupling wires side by side,
     # mixed within waves.  SCALED must give FAST's non-finite class on every
     # element and FAST's finite values to 1e-5 (per wire, relative to the
-    # wire's largest segment value), bit for bit where it falls back.
+    # wire's largest segment value), bit for bit where it falls back; both,
+    # and RECIP, the oracle's classes on every element and its finite values
+    # to 1e-5.
     c = O.generate(4, 256, 1024, 95, 41, O.PROFILE_STABLE)
     w, n = c.wires, c.nodes
     rng = np.random.default_rng(11)
@@ -98,10 +101,18 @@ def test_scaled_matches_fast_on_random_parameters(C, O):
     n["voltage"] = (rng.uniform(-1, 1, len(n)) * 10.0 ** rng.uniform(-2, 2, len(n))).astype(np.float32)
     scaled = _run_accessor(C, c, C.POLICY_SCALED)
     fast = _run_accessor(C, c, C.POLICY_FAST)
+    recip = _run_accessor(C, c, C.POLICY_RECIP)
+    ref = c.copy()
+    for p in c.pieces:
+        O.calc_new_currents(ref, p)
     for f in ("current", "voltage"):
         assert_same_class(scaled[f], fast[f], f)
         fin = np.isfinite(fast[f]).all(axis=1)
         assert max_wire_rel_err(scaled[f][fin], fast[f][fin]) <= TOL, f
+        fin = np.isfinite(ref.wires[f]).all(axis=1)
+        for name, got in (("scaled", scaled), ("fast", fast), ("recip", recip)):
+            assert_same_class(got[f], ref.wires[f], f"{name} {f} vs oracle")
+            assert max_wire_rel_err(got[f][fin], ref.wires[f][fin]) <= TOL, (name, f)
     # the regimes are all present
     dt = 1e-6
     R, L, Cw = (w[k].astype(np.float64) for k in ("resistance", "inductance", "capacitance"))
