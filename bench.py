@@ -371,9 +371,11 @@ def run_accessor(C, args):
     reference's GPU task wrappers call (circuit.cc:475-550) -- on the 64-piece
     circuit (4.19 M wires) in the reference layouts (128-byte wires, 16-byte
     nodes) in device memory: one call per piece per task, each waiting for its
-    kernel, as the reference runtime runs GPU point tasks.  On one GPU the private, shared and ghost
-    node instances are one array; shared and ghost charges go to fold buffers
-    that are applied once per loop (the runtime's reduction-instance apply).
+    kernel, as the reference runtime runs GPU point tasks; and the index-space
+    accessor entry points, one call per task over all pieces.  On one GPU the
+    private, shared and ghost node instances are one array; shared and ghost
+    charges go to fold buffers that are applied once per loop (the runtime's
+    reduction-instance apply).
     Beside it, the resident context in its per-piece mode and in its normal
     one-launch-per-task mode, on the same circuit."""
     import numpy as np
@@ -402,14 +404,25 @@ def run_accessor(C, args):
         for p in pieces:
             C.update_voltages_gpu(p, nodes, nodes, loc)
 
-    loop()
+    def loop_index():
+        C.calc_new_currents_index_gpu(pieces, dw, nodes, nodes, nodes, flag=policy)
+        C.fold_init_gpu(ofold, 0, N)
+        C.fold_init_gpu(gfold, 0, N)
+        C.distribute_charge_index_gpu(pieces, dw, nodes, ofold, gfold)
+        C.fold_apply_gpu(nodes, ofold, 0, N)
+        C.fold_apply_gpu(nodes, gfold, 0, N)
+        C.update_voltages_index_gpu(pieces, nodes, nodes, loc)
+
     loops = 3
-    t0 = time.perf_counter()
-    for _ in range(loops):
-        loop()
-    acc = W * loops / (time.perf_counter() - t0)
-    log(f"[accessor] {acc:.4g} wire-updates/s")
-    rows = {"accessor_form_per_piece_sync": acc}
+    rows = {}
+    for name, fn in (("accessor_form_per_piece_sync", loop),
+                     ("accessor_form_index_space", loop_index)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(loops):
+            fn()
+        rows[name] = W * loops / (time.perf_counter() - t0)
+        log(f"[accessor] {name}: {rows[name]:.4g} wire-updates/s")
     ctx = C.Context(0)
     ctx.generate(params)
     ctx.set_option(C.OPT_FUSE_DSC, 0)

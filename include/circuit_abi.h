@@ -210,6 +210,25 @@ int ckt_update_voltages_gpu(const ckt_piece_t* piece, ckt_node_t* pvt, ckt_node_
  * out_ptr is >= num_nodes_total.  *num_bad receives the count. */
 int ckt_sanity_check_wires_gpu(const ckt_piece_t* piece, const ckt_wire_t* wires,
                                uint32_t num_nodes_total, uint64_t* num_bad, void* stream);
+/* The three tasks above over a table of `num_pieces` pieces (host memory) in
+ * one launch each: an index-space launch of the point tasks
+ * (circuit.cc:250-251,276-277,298-299) on the same instances, with the
+ * results of one call per piece in piece order (distribute_charge's float
+ * atomics aside, whose order is unspecified in either form) and one wait at
+ * the end.  The per-piece calls are this form with a table of one.  Each
+ * call allocates a few bytes of device scratch per piece (the piece table and
+ * a bad-record counter) from the stream-ordered allocator on `stream` and
+ * releases it before returning. */
+int ckt_calc_new_currents_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                    ckt_wire_t* wires, const ckt_node_t* pvt,
+                                    const ckt_node_t* owned, const ckt_node_t* ghost, int flag,
+                                    void* stream);
+int ckt_distribute_charge_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                    const ckt_wire_t* wires, ckt_node_t* pvt, float* owned_fold,
+                                    float* ghost_fold, int flag, void* stream);
+int ckt_update_voltages_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces, ckt_node_t* pvt,
+                                  ckt_node_t* owned, const int32_t* locator, int flag,
+                                  void* stream);
 /* Reduction-instance semantics: fill a fold buffer with AccumulateCharge's
  * identity 0.0f (circuit.cc:30) and apply it to the owner instance with
  * AccumulateCharge::apply<true> (circuit.cc:32-36; lowlevel_dma.cc:324-358). */

@@ -147,6 +147,9 @@ def _declare(L):
         "ckt_generate_sizes": ([P(_GenParams), P(u32), P(u32), P(u32)], i32),
         "ckt_generate": ([P(_GenParams), vp, vp, vp, vp], i32),
         "ckt_calc_new_currents_gpu": ([vp, vp, vp, vp, vp, i32, vp], i32),
+        "ckt_calc_new_currents_index_gpu": ([vp, u32, vp, vp, vp, vp, i32, vp], i32),
+        "ckt_distribute_charge_index_gpu": ([vp, u32, vp, vp, vp, vp, i32, vp], i32),
+        "ckt_update_voltages_index_gpu": ([vp, u32, vp, vp, vp, i32, vp], i32),
         "ckt_distribute_charge_gpu": ([vp, vp, vp, vp, vp, i32, vp], i32),
         "ckt_update_voltages_gpu": ([vp, vp, vp, vp, i32, vp], i32),
         "ckt_sanity_check_wires_gpu": ([vp, vp, u32, P(u64), vp], i32),
@@ -328,6 +331,27 @@ def distribute_charge_gpu(piece, wires: DeviceArray, pvt: DeviceArray, owned_fol
 def update_voltages_gpu(piece, pvt: DeviceArray, owned: DeviceArray, locator: DeviceArray):
     pa = _piece_arg(piece)
     _check(lib().ckt_update_voltages_gpu(_p(pa), pvt.ptr, owned.ptr, locator.ptr, 0, None))
+
+
+# The same over a table of pieces in one launch each (index-space form).
+def calc_new_currents_index_gpu(pieces, wires: DeviceArray, pvt: DeviceArray, owned: DeviceArray,
+                                ghost: DeviceArray, flag=POLICY_EXACT):
+    pa = _piece_arg(pieces)
+    _check(lib().ckt_calc_new_currents_index_gpu(_p(pa), len(pa), wires.ptr, pvt.ptr, owned.ptr,
+                                                 ghost.ptr, flag, None))
+
+
+def distribute_charge_index_gpu(pieces, wires: DeviceArray, pvt: DeviceArray,
+                                owned_fold: DeviceArray, ghost_fold: DeviceArray):
+    pa = _piece_arg(pieces)
+    _check(lib().ckt_distribute_charge_index_gpu(_p(pa), len(pa), wires.ptr, pvt.ptr,
+                                                 owned_fold.ptr, ghost_fold.ptr, 0, None))
+
+
+def update_voltages_index_gpu(pieces, pvt: DeviceArray, owned: DeviceArray, locator: DeviceArray):
+    pa = _piece_arg(pieces)
+    _check(lib().ckt_update_voltages_index_gpu(_p(pa), len(pa), pvt.ptr, owned.ptr, locator.ptr, 0,
+                                               None))
 
 
 def sanity_check_wires_gpu(piece, wires: DeviceArray, num_nodes_total):

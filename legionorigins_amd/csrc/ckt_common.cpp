@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ckt_internal.h"
 
@@ -35,35 +36,91 @@ static int finish(hipError_t launched, hipStream_t s, const char* what) {
   return CKT_OK;
 }
 
-// Device counter for bad-record counts of the accessor-form kernels; one per
-// host thread and device, allocated on first use (so concurrent callers on
-// different threads never share one).
-static unsigned long long* bad_counter(int* status) {
-  static thread_local unsigned long long* ptrs[64] = {nullptr};
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess || dev < 0 || dev >= 64) {
-    *status = hip_fail(e == hipSuccess ? hipErrorInvalidDevice : e, "hipGetDevice");
-    return nullptr;
+// Device scratch of an accessor-form call over n pieces: the bad-record
+// counter (8 bytes, then 8 of padding), first_block[n + 1] (PieceIndex), and
+// the piece table on a 16-byte boundary.
+struct IndexLayout {
+  size_t first_block, pieces, bytes;
+  explicit IndexLayout(uint32_t n)
+      : first_block(16),
+        pieces((16 + 4 * ((size_t)n + 1) + 15) & ~(size_t)15),
+        bytes(pieces + sizeof(ckt_piece_t) * (size_t)n) {}
+};
+
+// Host image of the scratch's head: the zeroed counter (image[0..3]) and
+// first_block[] over each piece's wires or nodes (image[4..4+n]); n + 5 words.
+static int stage_index(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, uint32_t* image,
+                       uint32_t* blocks) {
+  image[0] = image[1] = image[2] = image[3] = 0;
+  uint64_t acc = 0;
+  for (uint32_t p = 0; p < n; p++) {
+    image[4 + p] = (uint32_t)acc;
+    const uint32_t count = over_nodes ? pieces[p].num_nodes : pieces[p].num_wires;
+    acc += (count + 255u) / 256u;
+    if (acc > 0x7fffffffu)
+      return fail(CKT_ERR_OUT_OF_RANGE, "accessor-form call: more than 2^31 workgroups");
   }
-  if (!ptrs[dev]) {
-    e = hipMalloc((void**)&ptrs[dev], sizeof(unsigned long long));
-    if (e != hipSuccess) {
-      *status = hip_fail(e, "hipMalloc(bad counter)");
-      ptrs[dev] = nullptr;
-      return nullptr;
-    }
-  }
-  *status = CKT_OK;
-  return ptrs[dev];
+  image[4 + n] = (uint32_t)acc;
+  *blocks = (uint32_t)acc;
+  return CKT_OK;
 }
 
-static int read_bad(unsigned long long* d_bad, hipStream_t s, uint64_t* out) {
-  unsigned long long h = 0;
-  CKT_HIP(hipMemcpyAsync(&h, d_bad, sizeof(h), hipMemcpyDeviceToHost, s));
-  CKT_HIP(hipStreamSynchronize(s));
-  *out = h;
+// One accessor-form task over a piece table, with its scratch at `dev`
+// (IndexLayout(n).bytes of device memory) and `image` (n + 5 words of host
+// memory) to stage it: copies the table in, launches, reads the bad-record
+// counter into *nbad (if not NULL) and waits, as the reference launchers do
+// (cudaDeviceSynchronize, circuit_gpu.cu:204).
+template <class Launch>
+static int run_indexed(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, unsigned char* dev,
+                       uint32_t* image, hipStream_t s, const char* what, uint64_t* nbad,
+                       Launch launch) {
+  const IndexLayout lay(n);
+  uint32_t blocks = 0;
+  int st = stage_index(pieces, n, over_nodes, image, &blocks);
+  if (st) return st;
+  const PieceIndex ix{(const ckt_piece_t*)(dev + lay.pieces),
+                      (const uint32_t*)(dev + lay.first_block), n};
+  unsigned long long* d_bad = (unsigned long long*)dev;
+  unsigned long long h_bad = 0;
+  hipError_t e = hipMemcpyAsync(dev, image, 4 * ((size_t)n + 5), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev + lay.pieces, pieces, sizeof(ckt_piece_t) * n, hipMemcpyHostToDevice,
+                       s);
+  if (e == hipSuccess) e = launch(ix, blocks, d_bad);
+  if (e == hipSuccess && nbad)
+    e = hipMemcpyAsync(&h_bad, d_bad, sizeof(h_bad), hipMemcpyDeviceToHost, s);
+  // The copies read and write host memory the caller owns: wait in every case.
+  const hipError_t w = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (w != hipSuccess) return hip_fail(w, what);
+  if (nbad) *nbad = h_bad;
   return CKT_OK;
+}
+
+// run_indexed with its scratch from the stream-ordered allocator, released
+// behind the call.
+template <class Launch>
+static int run_indexed_alloc(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, hipStream_t s,
+                             const char* what, uint64_t* nbad, Launch launch) {
+  unsigned char* dev = nullptr;
+  CKT_HIP(hipMallocAsync((void**)&dev, IndexLayout(n).bytes, s));
+  std::vector<uint32_t> image((size_t)n + 5);
+  const int st = run_indexed(pieces, n, over_nodes, dev, image.data(), s, what, nbad, launch);
+  const hipError_t e = hipFreeAsync(dev, s);
+  if (st) return st;
+  if (e != hipSuccess) return hip_fail(e, what);
+  return CKT_OK;
+}
+
+static int check_pieces(const ckt_piece_t* pieces, uint32_t n, const char* what) {
+  if (!pieces && n) return fail(CKT_ERR_INVALID_ARG, std::string(what) + ": null piece table");
+  return CKT_OK;
+}
+
+static int bad_locations(uint64_t nbad, const char* what) {
+  if (!nbad) return CKT_OK;
+  return fail(CKT_ERR_INVALID_ARG, std::string(what) + ": " + std::to_string(nbad) +
+                                       " wires with an invalid PointerLocation");
 }
 
 }  // namespace ckt
@@ -143,72 +200,97 @@ int ckt_device_synchronize(int device) {
 // ---------------------------------------------------------------------------
 // Accessor-form drop-ins.
 
+int ckt_calc_new_currents_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                    ckt_wire_t* wires, const ckt_node_t* pvt,
+                                    const ckt_node_t* owned, const ckt_node_t* ghost, int flag,
+                                    void* stream) {
+  const char* what = "calc_new_currents";
+  if (!wires || !pvt || !owned || !ghost)
+    return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: null instance");
+  if (int st = check_pieces(pieces, num_pieces, what)) return st;
+  if (flag < CKT_POLICY_EXACT || flag > CKT_POLICY_SCALED)
+    return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: flag is not a CKT_POLICY_*");
+  hipStream_t s = (hipStream_t)stream;
+  uint64_t nbad = 0;
+  int st = run_indexed_alloc(pieces, num_pieces, false, s, what, &nbad,
+                             [&](const PieceIndex& ix, uint32_t blocks, unsigned long long* bad) {
+                               return launch_cnc_aos(flag, ix, blocks, wires, pvt, owned, ghost,
+                                                     CKT_STEPS, bad, s);
+                             });
+  return st ? st : bad_locations(nbad, what);
+}
+
 int ckt_calc_new_currents_gpu(const ckt_piece_t* piece, ckt_wire_t* wires, const ckt_node_t* pvt,
                               const ckt_node_t* owned, const ckt_node_t* ghost, int flag,
                               void* stream) {
-  if (!piece || !wires || !pvt || !owned || !ghost)
-    return fail(CKT_ERR_INVALID_ARG, "ckt_calc_new_currents_gpu: null argument");
-  if (flag < CKT_POLICY_EXACT || flag > CKT_POLICY_SCALED)
-    return fail(CKT_ERR_INVALID_ARG, "ckt_calc_new_currents_gpu: flag is not a CKT_POLICY_*");
-  int st;
-  unsigned long long* bad = bad_counter(&st);
-  if (!bad) return st;
+  if (!piece) return fail(CKT_ERR_INVALID_ARG, "ckt_calc_new_currents_gpu: null argument");
+  return ckt_calc_new_currents_index_gpu(piece, 1, wires, pvt, owned, ghost, flag, stream);
+}
+
+int ckt_distribute_charge_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                    const ckt_wire_t* wires, ckt_node_t* pvt, float* owned_fold,
+                                    float* ghost_fold, int flag, void* stream) {
+  const char* what = "distribute_charge";
+  if (!wires || !pvt || !owned_fold || !ghost_fold)
+    return fail(CKT_ERR_INVALID_ARG, "distribute_charge: null instance");
+  if (int st = check_pieces(pieces, num_pieces, what)) return st;
+  if (flag != 0) return fail(CKT_ERR_INVALID_ARG, "distribute_charge: flag must be 0");
   hipStream_t s = (hipStream_t)stream;
-  CKT_HIP(hipMemsetAsync(bad, 0, sizeof(*bad), s));
-  st = finish(launch_cnc_aos(flag, piece->first_wire, piece->num_wires, wires, pvt, owned, ghost,
-                             CKT_STEPS, bad, s),
-              s, "calc_new_currents");
-  if (st) return st;
   uint64_t nbad = 0;
-  if ((st = read_bad(bad, s, &nbad))) return st;
-  if (nbad) return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: " + std::to_string(nbad) +
-                                                 " wires with an invalid PointerLocation");
-  return CKT_OK;
+  int st = run_indexed_alloc(pieces, num_pieces, false, s, what, &nbad,
+                             [&](const PieceIndex& ix, uint32_t blocks, unsigned long long* bad) {
+                               return launch_dsc_aos(ix, blocks, wires, pvt, owned_fold,
+                                                     ghost_fold, bad, s);
+                             });
+  return st ? st : bad_locations(nbad, what);
 }
 
 int ckt_distribute_charge_gpu(const ckt_piece_t* piece, const ckt_wire_t* wires, ckt_node_t* pvt,
                               float* owned_fold, float* ghost_fold, int flag, void* stream) {
-  if (!piece || !wires || !pvt || !owned_fold || !ghost_fold)
-    return fail(CKT_ERR_INVALID_ARG, "ckt_distribute_charge_gpu: null argument");
-  if (flag != 0) return fail(CKT_ERR_INVALID_ARG, "ckt_distribute_charge_gpu: flag must be 0");
-  int st;
-  unsigned long long* bad = bad_counter(&st);
-  if (!bad) return st;
+  if (!piece) return fail(CKT_ERR_INVALID_ARG, "ckt_distribute_charge_gpu: null argument");
+  return ckt_distribute_charge_index_gpu(piece, 1, wires, pvt, owned_fold, ghost_fold, flag,
+                                         stream);
+}
+
+int ckt_update_voltages_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces, ckt_node_t* pvt,
+                                  ckt_node_t* owned, const int32_t* locator, int flag,
+                                  void* stream) {
+  const char* what = "update_voltages";
+  if (!pvt || !owned || !locator)
+    return fail(CKT_ERR_INVALID_ARG, "update_voltages: null instance");
+  if (int st = check_pieces(pieces, num_pieces, what)) return st;
+  if (flag != 0) return fail(CKT_ERR_INVALID_ARG, "update_voltages: flag must be 0");
   hipStream_t s = (hipStream_t)stream;
-  CKT_HIP(hipMemsetAsync(bad, 0, sizeof(*bad), s));
-  st = finish(launch_dsc_aos(piece->first_wire, piece->num_wires, wires, pvt, owned_fold, ghost_fold,
-                             bad, s),
-              s, "distribute_charge");
-  if (st) return st;
-  uint64_t nbad = 0;
-  if ((st = read_bad(bad, s, &nbad))) return st;
-  if (nbad) return fail(CKT_ERR_INVALID_ARG, "distribute_charge: " + std::to_string(nbad) +
-                                                 " wires with an invalid PointerLocation");
-  return CKT_OK;
+  return run_indexed_alloc(pieces, num_pieces, true, s, what, nullptr,
+                           [&](const PieceIndex& ix, uint32_t blocks, unsigned long long*) {
+                             return launch_upv_aos(ix, blocks, pvt, owned, locator, s);
+                           });
 }
 
 int ckt_update_voltages_gpu(const ckt_piece_t* piece, ckt_node_t* pvt, ckt_node_t* owned,
                             const int32_t* locator, int flag, void* stream) {
-  if (!piece || !pvt || !owned || !locator)
-    return fail(CKT_ERR_INVALID_ARG, "ckt_update_voltages_gpu: null argument");
-  if (flag != 0) return fail(CKT_ERR_INVALID_ARG, "ckt_update_voltages_gpu: flag must be 0");
-  hipStream_t s = (hipStream_t)stream;
-  return finish(launch_upv_aos(piece->first_node, piece->num_nodes, pvt, owned, locator, s), s,
-                "update_voltages");
+  if (!piece) return fail(CKT_ERR_INVALID_ARG, "ckt_update_voltages_gpu: null argument");
+  return ckt_update_voltages_index_gpu(piece, 1, pvt, owned, locator, flag, stream);
 }
 
 int ckt_sanity_check_wires_gpu(const ckt_piece_t* piece, const ckt_wire_t* wires,
                                uint32_t num_nodes_total, uint64_t* num_bad, void* stream) {
   if (!piece || !wires || !num_bad)
     return fail(CKT_ERR_INVALID_ARG, "ckt_sanity_check_wires_gpu: null argument");
-  int st;
-  unsigned long long* bad = bad_counter(&st);
-  if (!bad) return st;
   hipStream_t s = (hipStream_t)stream;
-  CKT_HIP(hipMemsetAsync(bad, 0, sizeof(*bad), s));
-  st = finish(launch_sanity_aos(*piece, wires, num_nodes_total, bad, s), s, "sanity_check_wires");
-  if (st) return st;
-  return read_bad(bad, s, num_bad);
+  unsigned long long* bad = nullptr;
+  CKT_HIP(hipMallocAsync((void**)&bad, sizeof(*bad), s));
+  unsigned long long h = 0;
+  hipError_t e = hipMemsetAsync(bad, 0, sizeof(*bad), s);
+  if (e == hipSuccess) e = launch_sanity_aos(*piece, wires, num_nodes_total, bad, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, s);
+  const hipError_t f = hipFreeAsync(bad, s);
+  const hipError_t w = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "sanity_check_wires");
+  if (f != hipSuccess) return hip_fail(f, "sanity_check_wires");
+  if (w != hipSuccess) return hip_fail(w, "sanity_check_wires");
+  *num_bad = h;
+  return CKT_OK;
 }
 
 int ckt_fold_init_gpu(float* fold, uint32_t first, uint32_t count, void* stream) {

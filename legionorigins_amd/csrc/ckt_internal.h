@@ -76,14 +76,26 @@ hipError_t launch_sanity(const DevCircuit* circuits, const Tile* tiles, uint32_t
 hipError_t launch_nop(hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
-// Accessor-form (reference AoS layout) launchers.
-hipError_t launch_cnc_aos(int policy, uint32_t first_wire, uint32_t num_wires, ckt_wire_t* wires,
+// The piece table of an accessor-form launch, in device memory: pieces[p]
+// and first_block[p], the number of 256-element workgroups of the pieces
+// before p (first_block[num_pieces] = the launch's workgroup count), counted
+// over wires (calc_new_currents, distribute_charge) or nodes
+// (update_voltages).
+struct PieceIndex {
+  const ckt_piece_t* pieces;
+  const uint32_t* first_block;
+  uint32_t num_pieces;
+};
+
+// Accessor-form (reference AoS layout) launchers over a piece table;
+// `blocks` = first_block[num_pieces].
+hipError_t launch_cnc_aos(int policy, const PieceIndex& ix, uint32_t blocks, ckt_wire_t* wires,
                           const ckt_node_t* pvt, const ckt_node_t* owned, const ckt_node_t* ghost,
                           int steps, unsigned long long* bad, hipStream_t s);
-hipError_t launch_dsc_aos(uint32_t first_wire, uint32_t num_wires, const ckt_wire_t* wires,
+hipError_t launch_dsc_aos(const PieceIndex& ix, uint32_t blocks, const ckt_wire_t* wires,
                           ckt_node_t* pvt, float* owned_fold, float* ghost_fold,
                           unsigned long long* bad, hipStream_t s);
-hipError_t launch_upv_aos(uint32_t first_node, uint32_t num_nodes, ckt_node_t* pvt,
+hipError_t launch_upv_aos(const PieceIndex& ix, uint32_t blocks, ckt_node_t* pvt,
                           ckt_node_t* owned, const int32_t* locator, hipStream_t s);
 hipError_t launch_sanity_aos(const ckt_piece_t piece, const ckt_wire_t* wires,
                              uint32_t num_nodes_total, unsigned long long* bad, hipStream_t s);

@@ -824,15 +824,33 @@ __device__ __forceinline__ const ckt_node_t* pick_instance(int32_t loc, const ck
   }
 }
 
+// The accessor-form kernels run over a piece table (an index-space launch;
+// a per-piece call is a table of one).  Workgroups are numbered across the
+// pieces: piece p owns workgroups [first_block[p], first_block[p + 1]), each
+// covering 256 consecutive elements (wires or nodes) of it.
+__device__ __forceinline__ uint32_t piece_of_block(const PieceIndex& ix, uint32_t b,
+                                                   uint32_t* elem) {
+  // The last piece p with first_block[p] <= b (pieces without elements own
+  // no workgroup and are passed over).
+  uint32_t lo = 0, hi = ix.num_pieces;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ix.first_block[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  *elem = (b - ix.first_block[lo]) * 256u + threadIdx.x;
+  return lo;
+}
+
 template <int POLICY>
-__global__ __launch_bounds__(256) void cnc_aos_kernel(uint32_t first, uint32_t num_wires,
-                                                      ckt_wire_t* __restrict__ wires,
+__global__ __launch_bounds__(256) void cnc_aos_kernel(PieceIndex ix, ckt_wire_t* __restrict__ wires,
                                                       const ckt_node_t* pvt, const ckt_node_t* owned,
                                                       const ckt_node_t* ghost, int steps,
                                                       unsigned long long* bad) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= num_wires) return;
-  ckt_wire_t& wire = wires[first + tid];
+  uint32_t k;
+  const ckt_piece_t p = ix.pieces[piece_of_block(ix, blockIdx.x, &k)];
+  if (k >= p.num_wires) return;
+  ckt_wire_t& wire = wires[p.first_wire + k];
   const ckt_node_t* in_base = pick_instance(wire.in_loc, pvt, owned, ghost);
   const ckt_node_t* out_base = pick_instance(wire.out_loc, pvt, owned, ghost);
   if (!in_base || !out_base) {
@@ -862,13 +880,14 @@ __device__ __forceinline__ bool reduce_local(int32_t loc, uint32_t ptr, float v,
   }
 }
 
-__global__ __launch_bounds__(256) void dsc_aos_kernel(uint32_t first, uint32_t num_wires,
+__global__ __launch_bounds__(256) void dsc_aos_kernel(PieceIndex ix,
                                                       const ckt_wire_t* __restrict__ wires,
                                                       ckt_node_t* pvt, float* owned_fold,
                                                       float* ghost_fold, unsigned long long* bad) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= num_wires) return;
-  const ckt_wire_t& wire = wires[first + tid];
+  uint32_t k;
+  const ckt_piece_t p = ix.pieces[piece_of_block(ix, blockIdx.x, &k)];
+  if (k >= p.num_wires) return;
+  const ckt_wire_t& wire = wires[p.first_wire + k];
   const float dt = CKT_DT_F;
   bool ok = reduce_local(wire.in_loc, wire.in_ptr, -dt * wire.current[0], pvt, owned_fold, ghost_fold);
   ok = reduce_local(wire.out_loc, wire.out_ptr, dt * wire.current[kSeg - 1], pvt, owned_fold,
@@ -876,12 +895,13 @@ __global__ __launch_bounds__(256) void dsc_aos_kernel(uint32_t first, uint32_t n
   if (!ok) atomicAdd(bad, 1ull);
 }
 
-__global__ __launch_bounds__(256) void upv_aos_kernel(uint32_t first, uint32_t num_nodes,
-                                                      ckt_node_t* pvt, ckt_node_t* owned,
+__global__ __launch_bounds__(256) void upv_aos_kernel(PieceIndex ix, ckt_node_t* pvt,
+                                                      ckt_node_t* owned,
                                                       const int32_t* __restrict__ locator) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= num_nodes) return;
-  const uint32_t i = first + tid;
+  uint32_t k;
+  const ckt_piece_t p = ix.pieces[piece_of_block(ix, blockIdx.x, &k)];
+  if (k >= p.num_nodes) return;
+  const uint32_t i = p.first_node + k;
   ckt_node_t& nd = locator[i] ? pvt[i] : owned[i];
   float v = nd.voltage;
   v = v + nd.charge / nd.capacitance;
@@ -1051,43 +1071,42 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_cnc_aos(int policy, uint32_t first, uint32_t num, ckt_wire_t* wires,
+hipError_t launch_cnc_aos(int policy, const PieceIndex& ix, uint32_t blocks, ckt_wire_t* wires,
                           const ckt_node_t* pvt, const ckt_node_t* owned, const ckt_node_t* ghost,
                           int steps, unsigned long long* bad, hipStream_t s) {
-  if (num == 0) return hipSuccess;
-  const dim3 g(blocks_for(num, 256)), b(256);
+  if (blocks == 0) return hipSuccess;
+  const dim3 g(blocks), b(256);
   switch (policy) {
     case CKT_POLICY_EXACT:
-      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_EXACT>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_EXACT>, g, b, 0, s, ix, wires, pvt, owned, ghost, steps, bad);
       break;
     case CKT_POLICY_RECIP:
-      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_RECIP>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_RECIP>, g, b, 0, s, ix, wires, pvt, owned, ghost, steps, bad);
       break;
     case CKT_POLICY_FAST:
-      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_FAST>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_FAST>, g, b, 0, s, ix, wires, pvt, owned, ghost, steps, bad);
       break;
     case CKT_POLICY_SCALED:
-      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_SCALED>, g, b, 0, s, first, num, wires, pvt, owned, ghost, steps, bad);
+      hipLaunchKernelGGL(cnc_aos_kernel<CKT_POLICY_SCALED>, g, b, 0, s, ix, wires, pvt, owned, ghost, steps, bad);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_dsc_aos(uint32_t first, uint32_t num, const ckt_wire_t* wires, ckt_node_t* pvt,
-                          float* owned_fold, float* ghost_fold, unsigned long long* bad,
-                          hipStream_t s) {
-  if (num == 0) return hipSuccess;
-  hipLaunchKernelGGL(dsc_aos_kernel, dim3(blocks_for(num, 256)), dim3(256), 0, s, first, num, wires,
-                     pvt, owned_fold, ghost_fold, bad);
+hipError_t launch_dsc_aos(const PieceIndex& ix, uint32_t blocks, const ckt_wire_t* wires,
+                          ckt_node_t* pvt, float* owned_fold, float* ghost_fold,
+                          unsigned long long* bad, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(dsc_aos_kernel, dim3(blocks), dim3(256), 0, s, ix, wires, pvt, owned_fold,
+                     ghost_fold, bad);
   return hipGetLastError();
 }
 
-hipError_t launch_upv_aos(uint32_t first, uint32_t num, ckt_node_t* pvt, ckt_node_t* owned,
-                          const int32_t* locator, hipStream_t s) {
-  if (num == 0) return hipSuccess;
-  hipLaunchKernelGGL(upv_aos_kernel, dim3(blocks_for(num, 256)), dim3(256), 0, s, first, num, pvt,
-                     owned, locator);
+hipError_t launch_upv_aos(const PieceIndex& ix, uint32_t blocks, ckt_node_t* pvt,
+                          ckt_node_t* owned, const int32_t* locator, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(upv_aos_kernel, dim3(blocks), dim3(256), 0, s, ix, pvt, owned, locator);
   return hipGetLastError();
 }
 

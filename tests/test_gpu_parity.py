@@ -266,6 +266,65 @@ def test_accessor_form_dropins(C, O):
         C.calc_new_currents_gpu(c.pieces[0], dw, pvt, owned, ghost)
 
 
+def _uneven_pieces(O, W, N, cuts_w, cuts_n):
+    """A piece table over all W wires and N nodes in uneven ranges, with an
+    empty piece among them."""
+    bw = np.unique(np.clip(np.concatenate([[0], cuts_w, [W]]), 0, W))
+    bn = np.linspace(0, N, len(bw)).astype(np.int64)
+    for i, c in enumerate(cuts_n[:len(bn) - 2]):
+        bn[i + 1] = min(max(c, bn[i]), N)
+    pieces = np.zeros(len(bw) + 1, O.PIECE)
+    pieces[1:]["first_wire"], pieces[1:]["num_wires"] = bw, np.diff(np.append(bw, W))
+    pieces[1:]["first_node"], pieces[1:]["num_nodes"] = bn, np.diff(np.append(bn, N))
+    pieces[0] = (0, 0, 0, 0)
+    return pieces
+
+
+def test_accessor_index_form(C, O):
+    """The index-space accessor entry points over uneven pieces (sizes 0, 1,
+    255, 256, 257, ... elements, an empty piece first) give the per-piece
+    results: the relaxation and update bit-identical to the oracle, the
+    charge reduction within its float-atomic reordering."""
+    c = O.generate(4, 256, 1024, 70, 33, O.PROFILE_STABLE)
+    N, W = len(c.nodes), len(c.wires)
+    pieces = _uneven_pieces(O, W, N, [1, 256, 512, 513, 1300, 4000], [1, 2, 258, 600, 700])
+    assert pieces["num_wires"].sum() == W and pieces["num_nodes"].sum() == N
+    assert pieces["num_wires"][0] == 0 and 1 in pieces["num_wires"] and 256 in pieces["num_wires"]
+    dw = C.DeviceArray(c.wires)
+    pvt, owned, ghost = (C.DeviceArray(c.nodes) for _ in range(3))
+    ref = c.copy()
+    C.calc_new_currents_index_gpu(pieces, dw, pvt, owned, ghost, flag=C.POLICY_EXACT)
+    O.calc_new_currents(ref)
+    got = dw.download()
+    assert_bit_identical(got["current"], ref.wires["current"], "wire current")
+    assert_bit_identical(got["voltage"], ref.wires["voltage"], "wire voltage")
+    zeros = np.zeros(N, np.float32)
+    ofold, gfold = C.DeviceArray(zeros), C.DeviceArray(zeros)
+    C.distribute_charge_index_gpu(pieces, dw, pvt, ofold, gfold)
+    want_pvt, want_o, want_g = ref.nodes.copy(), zeros.copy(), zeros.copy()
+    O.distribute_charge_fold(ref.wires, want_pvt, want_o, want_g, 0, W)
+    assert max_rel_err(pvt.download()["charge"], want_pvt["charge"], floor=1e-12) <= TOL
+    assert max_rel_err(ofold.download(), want_o, floor=1e-12) <= TOL
+    assert max_rel_err(gfold.download(), want_g, floor=1e-12) <= TOL
+    pre = ref.nodes.copy()
+    pre["charge"] = np.random.default_rng(1).uniform(-1e-3, 1e-3, N).astype(np.float32)
+    pvt.upload(pre)
+    owned.upload(pre)
+    locator = C.DeviceArray(c.locator)
+    want_p, want_s = pre.copy(), pre.copy()
+    C.update_voltages_index_gpu(pieces, pvt, owned, locator)
+    O.update_voltages(ref, None, pvt=want_p, shr=want_s)
+    assert pvt.download().tobytes() == want_p.tobytes()
+    assert owned.download().tobytes() == want_s.tobytes()
+    # an empty table is a no-op; a bad location in any piece is reported
+    C.calc_new_currents_index_gpu(pieces[:0], dw, pvt, owned, ghost)
+    bad = got.copy()
+    bad["in_loc"][W - 1] = 7
+    dw.upload(bad)
+    with pytest.raises(C.CircuitError):
+        C.calc_new_currents_index_gpu(pieces, dw, pvt, owned, ghost)
+
+
 @pytest.mark.parametrize("flag", [1, 2, 3], ids=["recip", "fast", "scaled"])
 def test_accessor_form_fast_policies(C, O, flag):
     c = O.generate(4, 256, 1024, 70, 32, O.PROFILE_STABLE)
