@@ -229,6 +229,22 @@ int ckt_distribute_charge_index_gpu(const ckt_piece_t* pieces, uint32_t num_piec
 int ckt_update_voltages_index_gpu(const ckt_piece_t* pieces, uint32_t num_pieces, ckt_node_t* pvt,
                                   ckt_node_t* owned, const int32_t* locator, int flag,
                                   void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ckt_calc_new_currents_index_gpu without any allocation.
+ * Its scratch (the piece table and the bad-record counter) is a static device
+ * array of CKT_NOALLOC_SCRATCH_BYTES, one per device, held until the process
+ * ends; a table of more than CKT_NOALLOC_MAX_PIECES pieces is rejected with
+ * CKT_ERR_OUT_OF_RANGE.  Every call on a device shares that device's array
+ * (each call rewrites it, counter included, before its launch): calls that
+ * overlap on one device, from several host threads or streams, race, and the
+ * caller must not overlap them. */
+#define CKT_NOALLOC_MAX_PIECES 4096
+#define CKT_NOALLOC_SCRATCH_BYTES 81952
+int ckt_calc_new_currents_index_gpu_noalloc(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                            ckt_wire_t* wires, const ckt_node_t* pvt,
+                                            const ckt_node_t* owned, const ckt_node_t* ghost,
+                                            int flag, void* stream);
 /* Reduction-instance semantics: fill a fold buffer with AccumulateCharge's
  * identity 0.0f (circuit.cc:30) and apply it to the owner instance with
  * AccumulateCharge::apply<true> (circuit.cc:32-36; lowlevel_dma.cc:324-358). */

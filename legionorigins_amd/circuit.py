@@ -148,6 +148,7 @@ def _declare(L):
         "ckt_generate": ([P(_GenParams), vp, vp, vp, vp], i32),
         "ckt_calc_new_currents_gpu": ([vp, vp, vp, vp, vp, i32, vp], i32),
         "ckt_calc_new_currents_index_gpu": ([vp, u32, vp, vp, vp, vp, i32, vp], i32),
+        "ckt_calc_new_currents_index_gpu_noalloc": ([vp, u32, vp, vp, vp, vp, i32, vp], i32),
         "ckt_distribute_charge_index_gpu": ([vp, u32, vp, vp, vp, vp, i32, vp], i32),
         "ckt_update_voltages_index_gpu": ([vp, u32, vp, vp, vp, i32, vp], i32),
         "ckt_distribute_charge_gpu": ([vp, vp, vp, vp, vp, i32, vp], i32),

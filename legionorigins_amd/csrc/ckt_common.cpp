@@ -1,9 +1,10 @@
 // ckt_common.cpp -- status strings, device utilities and the accessor-form
 // drop-ins (the entry points of apps/circuit/circuit.h:83-104).
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,12 +40,16 @@ static int finish(hipError_t launched, hipStream_t s, const char* what) {
 // Device scratch of an accessor-form call over n pieces: the bad-record
 // counter (8 bytes, then 8 of padding), first_block[n + 1] (PieceIndex), and
 // the piece table on a 16-byte boundary.
+constexpr size_t index_pieces_offset(uint32_t n) {
+  return (16 + 4 * ((size_t)n + 1) + 15) & ~(size_t)15;
+}
+constexpr size_t index_bytes(uint32_t n) {
+  return index_pieces_offset(n) + sizeof(ckt_piece_t) * (size_t)n;
+}
 struct IndexLayout {
   size_t first_block, pieces, bytes;
   explicit IndexLayout(uint32_t n)
-      : first_block(16),
-        pieces((16 + 4 * ((size_t)n + 1) + 15) & ~(size_t)15),
-        bytes(pieces + sizeof(ckt_piece_t) * (size_t)n) {}
+      : first_block(16), pieces(index_pieces_offset(n)), bytes(index_bytes(n)) {}
 };
 
 // Host image of the scratch's head: the zeroed counter (image[0..3]) and
@@ -109,6 +114,35 @@ static int run_indexed_alloc(const ckt_piece_t* pieces, uint32_t n, bool over_no
   const hipError_t e = hipFreeAsync(dev, s);
   if (st) return st;
   if (e != hipSuccess) return hip_fail(e, what);
+  return CKT_OK;
+}
+
+// Device scratch of ckt_calc_new_currents_index_gpu_noalloc: the layout
+// above for CKT_NOALLOC_MAX_PIECES pieces, one copy per device (each GPU the
+// library runs on holds it until the process ends).  In host code `scratch`
+// names no device memory; kernels and copies get the per-device address that
+// hipGetSymbolAddress returns, looked up once per device on the first call
+// made with that device current.
+static_assert(index_bytes(CKT_NOALLOC_MAX_PIECES) == CKT_NOALLOC_SCRATCH_BYTES,
+              "CKT_NOALLOC_SCRATCH_BYTES is the layout's size for CKT_NOALLOC_MAX_PIECES");
+static __device__ __align__(256) unsigned char scratch[CKT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_status[kMaxDevices];
+static std::once_flag g_scratch_once[kMaxDevices];
+
+static int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  CKT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(CKT_ERR_NO_DEVICE, "allocation-free call: device index beyond 63");
+  std::call_once(g_scratch_once[dev], [dev] {
+    g_scratch_status[dev] = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+  });
+  if (g_scratch_status[dev] != hipSuccess)
+    return hip_fail(g_scratch_status[dev], "hipGetSymbolAddress(scratch)");
+  *out = (unsigned char*)g_scratch_ptr[dev];
   return CKT_OK;
 }
 
@@ -217,6 +251,34 @@ int ckt_calc_new_currents_index_gpu(const ckt_piece_t* pieces, uint32_t num_piec
                                return launch_cnc_aos(flag, ix, blocks, wires, pvt, owned, ghost,
                                                      CKT_STEPS, bad, s);
                              });
+  return st ? st : bad_locations(nbad, what);
+}
+
+int ckt_calc_new_currents_index_gpu_noalloc(const ckt_piece_t* pieces, uint32_t num_pieces,
+                                            ckt_wire_t* wires, const ckt_node_t* pvt,
+                                            const ckt_node_t* owned, const ckt_node_t* ghost,
+                                            int flag, void* stream) {
+  const char* what = "calc_new_currents";
+  if (!wires || !pvt || !owned || !ghost)
+    return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: null instance");
+  if (int st = check_pieces(pieces, num_pieces, what)) return st;
+  if (flag < CKT_POLICY_EXACT || flag > CKT_POLICY_SCALED)
+    return fail(CKT_ERR_INVALID_ARG, "calc_new_currents: flag is not a CKT_POLICY_*");
+  if (num_pieces > CKT_NOALLOC_MAX_PIECES)
+    return fail(CKT_ERR_OUT_OF_RANGE,
+                "ckt_calc_new_currents_index_gpu_noalloc: " + std::to_string(num_pieces) +
+                    " pieces, more than CKT_NOALLOC_MAX_PIECES (" +
+                    std::to_string(CKT_NOALLOC_MAX_PIECES) + ")");
+  unsigned char* dev = nullptr;
+  if (int st = noalloc_scratch(&dev)) return st;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t image[CKT_NOALLOC_MAX_PIECES + 5];
+  uint64_t nbad = 0;
+  int st = run_indexed(pieces, num_pieces, false, dev, image, s, what, &nbad,
+                       [&](const PieceIndex& ix, uint32_t blocks, unsigned long long* bad) {
+                         return launch_cnc_aos(flag, ix, blocks, wires, pvt, owned, ghost,
+                                               CKT_STEPS, bad, s);
+                       });
   return st ? st : bad_locations(nbad, what);
 }
 
