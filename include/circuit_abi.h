@@ -430,7 +430,8 @@ typedef struct ckt_request {
 } ckt_request_t;
 
 typedef struct ckt_request_result {
-  int32_t status;              /* CKT_OK, or why the request failed */
+  int32_t status;              /* CKT_OK, or why the request failed (CKT_ERR_CANCELLED,
+                                  CKT_ERR_TIMEOUT, ...) */
   uint32_t num_wires;
   int32_t num_loops;
   double arrival_s, admit_s, finish_s; /* seconds from trace start */
@@ -449,13 +450,46 @@ typedef struct ckt_server_report {
 
 int ckt_server_create(int device, const ckt_server_config_t* config, ckt_server_t** out);
 int ckt_server_destroy(ckt_server_t* s);
-/* Replay a request trace in real time: each request becomes visible at its
- * arrival offset, is admitted when budget and slots allow, and runs by
- * continuous batching.  results[i] corresponds to reqs[i]. */
+/* Online serving.  ckt_server_start launches the scheduler on a host thread
+ * of its own; requests then arrive by ckt_server_submit, from any host thread
+ * at any time, and are queued, admitted when the budget and caps allow,
+ * generated on the device and advanced by continuous batching.  Times in
+ * results are seconds since the start.  ckt_server_stop(s, drain) ends the
+ * run: with drain != 0 after every submitted request has finished, otherwise
+ * after cancelling the unfinished ones.  submit, poll, wait and cancel may be
+ * called concurrently with each other and with the scheduler. */
+int ckt_server_start(ckt_server_t* s);
+int ckt_server_stop(ckt_server_t* s, int drain);
+/* Queue a request; *id numbers requests from 0 in submission order since the
+ * start.  A request that can never run (invalid sizes, a negative loop count,
+ * a footprint above the budget) is refused with the status saying why.
+ * timeout_s > 0: not admitted within that many seconds of its submission ->
+ * it ends with CKT_ERR_TIMEOUT. */
+int ckt_server_submit(ckt_server_t* s, const ckt_gen_params_t* gen, int32_t num_loops,
+                      double timeout_s, uint32_t* id);
+/* *finished = 1 and *result filled once request `id` has ended (result->status
+ * says how); *finished = 0 while it is queued or running. */
+int ckt_server_poll(ckt_server_t* s, uint32_t id, int* finished, ckt_request_result_t* result);
+/* Wait for request `id` to end (timeout_s < 0: without limit); CKT_ERR_TIMEOUT
+ * if it has not ended within timeout_s. */
+int ckt_server_wait(ckt_server_t* s, uint32_t id, double timeout_s, ckt_request_result_t* result);
+/* Cancel request `id`: a queued request leaves the queue, a running one leaves
+ * at the next iteration boundary; either ends with CKT_ERR_CANCELLED.  A
+ * request whose last loop has already run when the scheduler sees the cancel
+ * ends normally.  CKT_ERR_STATE if it had already ended. */
+int ckt_server_cancel(ckt_server_t* s, uint32_t id);
+/* Replay a request trace in real time on the same scheduler (started and
+ * stopped by this call): each request is submitted at its arrival offset,
+ * with that offset as its arrival time.  results[i] corresponds to reqs[i]. */
 int ckt_server_run_trace(ckt_server_t* s, const ckt_request_t* reqs, uint32_t n,
                          ckt_request_result_t* results, ckt_server_report_t* report);
-/* Final state of request i of the last trace (kept when keep_state != 0). */
+/* Keep each finished request's final state for ckt_server_fetch (set before
+ * a start or a trace); after a trace `index` is the trace index, otherwise
+ * the request id. */
 int ckt_server_set_keep_state(ckt_server_t* s, int keep_state);
+/* Keep only the states of requests with at most max_wires wires (default:
+ * all). */
+int ckt_server_set_keep_limit(ckt_server_t* s, uint32_t max_wires);
 int ckt_server_fetch(ckt_server_t* s, uint32_t index, ckt_node_t* nodes, ckt_wire_t* wires);
 
 /* HBM calibration kernel: copies `bytes` from src to dst on the device with a

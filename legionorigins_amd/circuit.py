@@ -200,6 +200,13 @@ def _declare(L):
         "ckt_server_run_trace": ([vp, P(_Request), u32, P(_RequestResult), P(_ServerReport)], i32),
         "ckt_server_set_keep_state": ([vp, i32], i32),
         "ckt_server_fetch": ([vp, u32, vp, vp], i32),
+        "ckt_server_set_keep_limit": ([vp, u32], i32),
+        "ckt_server_start": ([vp], i32),
+        "ckt_server_stop": ([vp, i32], i32),
+        "ckt_server_submit": ([vp, P(_GenParams), i32, ctypes.c_double, P(u32)], i32),
+        "ckt_server_poll": ([vp, u32, P(i32), P(_RequestResult)], i32),
+        "ckt_server_wait": ([vp, u32, ctypes.c_double, P(_RequestResult)], i32),
+        "ckt_server_cancel": ([vp, u32], i32),
         "ckt_copy_kernel": ([vp, vp, sz, vp], i32),
     }
     for name, (args, res) in sig.items():
@@ -606,8 +613,9 @@ class Request:
 
 
 class Server:
-    """Continuous batching of circuit simulations (replays a request trace
-    in real time; see ckt_server_run_trace)."""
+    """Continuous batching of circuit simulations: online (start, submit,
+    poll, wait, cancel, stop) or replaying a request trace in real time
+    (run_trace)."""
 
     def __init__(self, device=0, budget_bytes=0, max_active_wires=0, max_active=0,
                  policy=POLICY_FAST, mode=REDUCE_ATOMIC, gen_threads=8):
@@ -626,8 +634,43 @@ class Server:
         except Exception:
             pass
 
-    def set_keep_state(self, keep=True):
+    def set_keep_state(self, keep=True, max_wires=None):
         _check(lib().ckt_server_set_keep_state(self.h, int(keep)))
+        _check(lib().ckt_server_set_keep_limit(self.h, 0xFFFFFFFF if max_wires is None
+                                               else int(max_wires)))
+
+    # Online serving (ckt_server_start/submit/poll/wait/cancel/stop).
+    def start(self):
+        _check(lib().ckt_server_start(self.h))
+
+    def stop(self, drain=True):
+        _check(lib().ckt_server_stop(self.h, int(drain)))
+
+    def submit(self, gen: GenParams, num_loops, timeout_s=0.0):
+        rid = ctypes.c_uint32()
+        g = gen.c()
+        _check(lib().ckt_server_submit(self.h, ctypes.byref(g), num_loops, timeout_s,
+                                       ctypes.byref(rid)))
+        return rid.value
+
+    @staticmethod
+    def _result(x):
+        return {"status": x.status, "num_wires": x.num_wires, "num_loops": x.num_loops,
+                "arrival_s": x.arrival_s, "admit_s": x.admit_s, "finish_s": x.finish_s,
+                "checksum": x.final_voltage_checksum}
+
+    def poll(self, rid):
+        done, res = ctypes.c_int(), _RequestResult()
+        _check(lib().ckt_server_poll(self.h, rid, ctypes.byref(done), ctypes.byref(res)))
+        return bool(done.value), self._result(res)
+
+    def wait(self, rid, timeout_s=-1.0):
+        res = _RequestResult()
+        _check(lib().ckt_server_wait(self.h, rid, timeout_s, ctypes.byref(res)))
+        return self._result(res)
+
+    def cancel(self, rid):
+        _check(lib().ckt_server_cancel(self.h, rid))
 
     def run_trace(self, requests):
         n = len(requests)
@@ -637,9 +680,7 @@ class Server:
         res = (_RequestResult * n)()
         rep = _ServerReport()
         _check(lib().ckt_server_run_trace(self.h, reqs, n, res, ctypes.byref(rep)))
-        results = [{"status": x.status, "num_wires": x.num_wires, "num_loops": x.num_loops,
-                    "arrival_s": x.arrival_s, "admit_s": x.admit_s, "finish_s": x.finish_s,
-                    "checksum": x.final_voltage_checksum} for x in res]
+        results = [self._result(x) for x in res]
         report = {k: getattr(rep, k) for k, _ in _ServerReport._fields_}
         return results, report
 

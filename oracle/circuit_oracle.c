@@ -522,14 +522,15 @@ static void* cnc_worker(void* arg) {
         orc_calc_new_currents_f64(j->seeding, j->steps, j->wires64, j->nodes64, p->first_wire,
                                   p->num_wires);
     } else {
-      /* chunks of `grain` wires over the concatenated pieces */
-      uint64_t lo = item * j->grain, base = 0;
+      /* chunks of `grain` wires over the concatenated pieces; a chunk that
+         crosses the end of a piece continues in the next one */
+      uint64_t lo = item * j->grain, hi = lo + j->grain, base = 0;
       int found = 0;
-      for (uint32_t p = 0; p < j->num_pieces; p++) {
+      for (uint32_t p = 0; p < j->num_pieces && lo < hi; p++) {
         uint64_t nw = j->pieces[p].num_wires;
         if (lo < base + nw) {
           uint64_t off = lo - base;
-          uint64_t cnt = nw - off < j->grain ? nw - off : j->grain;
+          uint64_t cnt = nw - off < hi - lo ? nw - off : hi - lo;
           uint32_t fw = j->pieces[p].first_wire + (uint32_t)off;
           if (j->prec == 32)
             orc_calc_new_currents_f32(j->seeding, j->steps, j->wires32, j->nodes32, j->nodes32,
@@ -539,8 +540,8 @@ static void* cnc_worker(void* arg) {
           else
             orc_calc_new_currents_f64(j->seeding, j->steps, j->wires64, j->nodes64, fw,
                                       (uint32_t)cnt);
+          lo += cnt;
           found = 1;
-          break;
         }
         base += nw;
       }

@@ -1,9 +1,12 @@
 """Batching on the GPU: circuits of uneven size advanced by shared launches
 give the same state as each circuit run alone (EXACT or SCALED with CSR:
-bit-identical; ATOMIC: within 1e-5)."""
+bit-identical; ATOMIC: within 1e-5) and as the oracle's loop, up to the full
+batched workload (512 circuits of 2^10..2^18 wires)."""
+import numpy as np
 import pytest
 
-from helpers import TOL, assert_bit_identical, max_rel_err
+from helpers import (TOL, assert_bit_identical, assert_same_class, max_rel_err,
+                     max_wire_rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -117,3 +120,74 @@ def test_atomic_implementations_agree(C):
         csr = ctx.store()[0]
     for o in outs:
         assert max_rel_err(o["voltage"], csr["voltage"]) <= TOL
+
+
+def _c4_shapes(C, profile):
+    # The batched workload: 512 circuits, wires log-uniform in [2^10, 2^18],
+    # nodes = wires/4, pieces = ceil(nodes/4096), 95% in-piece, seeds 1000+i.
+    rng = np.random.default_rng(4)
+    return [C.shape_for_wires(int(np.exp(rng.uniform(np.log(2**10), np.log(2**18)))), 95,
+                              1000 + i, profile) for i in range(512)]
+
+
+@pytest.mark.parametrize("profile", [0, 1], ids=["reference", "stable"])
+def test_batched_workload_512_circuits(C, O, profile):
+    # The full batched workload in one batch, 5 loops, with the benchmark's
+    # schedule (autotuned) and configuration (SCALED, fused ATOMIC), and in
+    # the stable profile also EXACT + CSR.  Every circuit runs its 5 loops;
+    # the ten smallest are checked against the oracle's 5-loop run: EXACT +
+    # CSR bit for bit, SCALED + ATOMIC by non-finite class on every node
+    # voltage, wire current and internal voltage and within 1e-5 where finite.
+    shapes = _c4_shapes(C, profile)
+    sizes = [(s.num_pieces * s.nodes_per_piece, s.num_pieces * s.wires_per_piece) for s in shapes]
+    smallest = sorted(range(len(shapes)), key=lambda i: sizes[i][1])[:10]
+    refs = {}
+    for i in smallest:
+        g = shapes[i]
+        ref = O.generate(g.num_pieces, g.nodes_per_piece, g.wires_per_piece,
+                         g.pct_wire_in_piece, g.random_seed, g.profile)
+        O.run(ref, 5)
+        refs[i] = ref
+    runs = [(C.POLICY_SCALED, C.REDUCE_ATOMIC)]
+    if profile == 1:
+        runs.append((C.POLICY_EXACT, C.REDUCE_CSR))
+    for policy, mode in runs:
+        b = C.Batch(0)
+        ids = b.add_generated_many(shapes, [5] * len(shapes), threads=8)
+        best, _ = b.autotune(policy, probe_steps=200)
+        best.policy, best.reduce_mode, best.use_graph = policy, mode, 1
+        rep = b.run(best)
+        assert rep["loops_run"] == 5
+        assert rep["wire_updates"] == 5 * sum(w for _, w in sizes)
+        assert all(b.loops_left(cid) == 0 for cid in ids)
+        for i in smallest:
+            N, W = sizes[i]
+            nodes, wires = b.store(ids[i], N, W)
+            ref = refs[i]
+            if policy == C.POLICY_EXACT:
+                assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], f"{i} node voltage")
+                assert_bit_identical(wires["current"], ref.wires["current"], f"{i} wire current")
+                continue
+            assert_same_class(nodes["voltage"], ref.nodes["voltage"], f"{i} node voltage")
+            assert_same_class(wires["current"], ref.wires["current"], f"{i} wire current")
+            assert_same_class(wires["voltage"], ref.wires["voltage"], f"{i} wire voltage")
+            assert max_rel_err(nodes["voltage"], ref.nodes["voltage"]) <= TOL
+            fin = np.isfinite(ref.wires["current"]).all(axis=1)
+            assert max_wire_rel_err(wires["current"][fin], ref.wires["current"][fin]) <= TOL
+        b.close()
+
+
+def test_batch_small_circuits_equal_oracle(C, O):
+    # EXACT + CSR in a batch against the oracle's loop on the same circuits
+    # (one-piece circuits with the wrap rule among them).
+    b = C.Batch(0)
+    ids = [b.add_generated(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE), loops)
+           for (P, npp, wpp, pct, seed, loops) in SHAPES]
+    b.run(C.Batch.config(policy=C.POLICY_EXACT, mode=C.REDUCE_CSR))
+    for cid, (P, npp, wpp, pct, seed, loops) in zip(ids, SHAPES):
+        nodes, wires = b.store(cid, P * npp, P * wpp)
+        ref = O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE)
+        O.run(ref, loops)
+        assert nodes.tobytes() == ref.nodes.tobytes()
+        assert wires.tobytes() == ref.wires.tobytes()
+    b.close()

@@ -17,6 +17,7 @@ update_voltages (circuit.cc:1290-1307)
 import math
 
 import numpy as np
+import pytest
 
 f32 = np.float32
 DT32 = f32(1e-6)
@@ -242,3 +243,35 @@ def test_update_voltages_special_cases(O):
     assert not np.where(priv, pvt["charge"], shr["charge"]).any()
     assert np.array_equal(np.where(priv, shr["voltage"], pvt["voltage"]),
                           np.where(priv, before["voltage"], before["voltage"]))
+
+
+@pytest.mark.parametrize("grain,threads", [(0, 1), (0, 8), (64, 1), (64, 8), (7, 3)])
+def test_loop_driver_equals_task_bodies_on_ragged_pieces(O, grain, threads):
+    # The threaded loop driver against the three task bodies called one after
+    # another (circuit.cc:232-307), on pieces whose wire counts are not
+    # multiples of the work chunk (so chunks cross piece boundaries), for both
+    # the FP32 run and its FP64 twin: every wire relaxed exactly once per loop.
+    a = O.generate(3, 61, 237, 90, 41, O.PROFILE_STABLE)
+    b = a.copy()
+    c64 = a.copy()
+    O.lib().orc_run_f32(0, 40, 3, 0, threads, grain, O._p(a.nodes), a.num_nodes, O._p(a.wires),
+                        a.num_wires, O._p(a.locator), O._p(a.pieces), len(a.pieces))
+    for _ in range(3):
+        for p in b.pieces:
+            O.calc_new_currents(b, p, steps=40)
+        for p in b.pieces:
+            O.distribute_charge(b, p)
+        for p in b.pieces:
+            O.update_voltages(b, p)
+    assert a.wires.tobytes() == b.wires.tobytes()
+    assert a.nodes.tobytes() == b.nodes.tobytes()
+    n64, w64 = O.run_f64(c64, 3, steps=40, nthreads=threads)
+    m64, x64 = O.promote(c64)
+    for _ in range(3):
+        O.calc_new_currents_f64(m64, x64, 0, c64.num_wires, steps=40)
+        for p in c64.pieces:
+            O.lib().orc_distribute_charge_f64(O._p(x64), O._p(m64), int(p["first_wire"]),
+                                              int(p["num_wires"]))
+        for p in c64.pieces:
+            O.lib().orc_update_voltages_f64(O._p(m64), int(p["first_node"]), int(p["num_nodes"]))
+    assert w64.tobytes() == x64.tobytes() and n64.tobytes() == m64.tobytes()
