@@ -11,9 +11,14 @@ calc_new_currents; --policy selects EXACT (bit-identical to the oracle), RECIP
 or FAST, and `detail.policies` times all four.
 
 Printed: one JSON line (rank 0) with the metric, a `roofline` object for the
-dominant kernel (calc_new_currents, FP32 VALU-bound) from HIP-event timing on
-the launch stream, and a `cpu_baseline` object: the CPU oracle's restatement
-of the reference task body timed on this host's cores on a bounded sample.
+dominant kernel (calc_new_currents, FP32 VALU-bound: executed flops over the
+HIP-event time on the launch stream), a `cpu_baseline` object (the CPU
+oracle's restatement of the reference task bodies, one loop over whole pieces,
+one piece per host thread) and `detail`: per-policy and per-reduction timings,
+the batched workload (512 circuits in shared launches against sequential
+launches), the served workload (Poisson 200 requests/s for 60 s: p50/p99
+latency and throughput against one request at a time with plain launches) and
+the FP64 error of each workload.
 """
 from __future__ import annotations
 
@@ -38,6 +43,9 @@ FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
 #   FAST   20 FMA + 8 add = 48
 #   SCALED 10 FMA + 6 add, plus 10 multiplies per 4 steps = 28.5
 EXECUTED_FLOPS = {"exact": 67, "recip": 77, "fast": 48, "scaled": 28.5}
+# baseline.cu on one Tesla M2090, 48 x 2500 x 16384, 10 loops, 181.033 s
+# (apps/circuit/keeneland_results/baseline_48_1_1_1.stdio).
+BASELINE_WIRE_UPDATES_PER_S = 48 * 16384 * 10 / 181.033
 
 
 def log(msg):
@@ -57,9 +65,11 @@ def parse():
     ap.add_argument("--profile", default="reference", choices=["reference", "stable"],
                     help="generator profile of the timed circuit (stable: finite values)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the per-policy and per-kernel detail runs")
+                    help="skip every detail run (policies, reductions, batched, served, FP64)")
+    ap.add_argument("--skip-batched", action="store_true", help="detail without the batched run")
+    ap.add_argument("--skip-served", action="store_true", help="detail without the served run")
+    ap.add_argument("--skip-fp64", action="store_true", help="detail without the FP64 errors")
     ap.add_argument("--batched", action="store_true",
                     help="batched workload: 512 circuits in shared launches vs sequential")
     ap.add_argument("--served", action="store_true",
@@ -105,29 +115,38 @@ def time_loops(C, ctx, loops, policy, mode):
     return rep, wall, k
 
 
-def cpu_baseline(C, params, seconds):
-    """The oracle's calc_new_currents (restating circuit.cc:352-413) on a
-    sample of the workload's wires, one wire-update per sampled wire."""
+def host_threads():
+    """Every core in this process's affinity mask, capped by OMP_NUM_THREADS
+    where the host sets it (a GPU box's share of its CPUs)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
+def cpu_baseline(C, params):
+    """One loop of the three task bodies, as the oracle restates them
+    (circuit.cc:352-460; loop order :232-307), over whole pieces of the
+    workload, one piece per thread (the reference's SharedMapper puts one
+    point task on each CPU, circuit.cc:623-635): calc_new_currents on
+    min(threads, pieces) pieces in parallel, then distribute_charge and
+    update_voltages over those pieces."""
     import numpy as np
     from oracle import oracle as O
     host = C.generate(params)
-    threads = min(16, os.cpu_count() or 1)
-    rng = np.random.default_rng(0)
-    # calibrate on a small sample, then size the timed sample to `seconds`
-    probe = np.sort(rng.choice(len(host.wires), 64 * threads, replace=False)).astype(np.uint32)
-    t = O.lib().orc_cnc_sample_f32(0, O.STEPS, threads, O._p(host.wires), O._p(host.nodes),
-                                   O._p(probe), len(probe))
-    per_wire = t / len(probe)
-    count = int(max(len(probe), min(len(host.wires), seconds / per_wire)))
-    idx = np.sort(rng.choice(len(host.wires), count, replace=False)).astype(np.uint32)
-    t = O.lib().orc_cnc_sample_f32(0, O.STEPS, threads, O._p(host.wires), O._p(host.nodes),
-                                   O._p(idx), len(idx))
-    return {"value": count / t, "unit": "wire-updates/s", "cores": threads, "kind": "port",
-            "sample": (f"calc_new_currents (10000 inner steps, FP32, reference operation order) "
-                       f"of {count} randomly sampled wires of the workload, one loop, "
-                       f"{threads} threads (one chunk of 16 wires per task); "
-                       f"distribute_charge/update_voltages (<0.01% of the work) not timed; "
-                       f"{t:.1f} s wall"),
+    threads = host_threads()
+    k = min(threads, len(host.pieces))
+    pieces = np.ascontiguousarray(host.pieces[:k])
+    W = int(pieces["num_wires"].sum())
+    t0 = time.perf_counter()
+    O.lib().orc_run_f32(0, O.STEPS, 1, 0, threads, 0, O._p(host.nodes), len(host.nodes),
+                        O._p(host.wires), len(host.wires), O._p(host.locator), O._p(pieces), k)
+    t = time.perf_counter() - t0
+    return {"value": W / t, "unit": "wire-updates/s", "cores": threads, "kind": "port",
+            "sample": (f"one loop of calc_new_currents (10000 inner steps, FP32, reference "
+                       f"operation order), distribute_charge and update_voltages over {k} of "
+                       f"the workload's {len(host.pieces)} pieces ({W} wires), one piece per "
+                       f"thread on {threads} threads (affinity mask: "
+                       f"{len(os.sched_getaffinity(0))} CPUs); {t:.1f} s wall"),
             "seconds": t}
 
 
@@ -146,14 +165,15 @@ def policy_of(C, name):
             "scaled": C.POLICY_SCALED}[name]
 
 
-def run_batched(C, args):
-    loops = 5
-    pol = policy_of(C, args.policy)
+def batched_measure(C, policy_name, loops=5):
+    """The batched workload in shared launches against the same circuits run
+    one after another."""
+    pol = policy_of(C, policy_name)
     shapes = batched_shapes(C)
     W_total = sum(s.num_pieces * s.wires_per_piece for s in shapes)
     b = C.Batch(0)
     t0 = time.perf_counter()
-    b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, os.cpu_count() or 1))
+    b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, host_threads()))
     log(f"[batched] {len(shapes)} circuits, {W_total} wires, added in "
         f"{time.perf_counter() - t0:.1f} s")
     best, probe_ms = b.autotune(pol, probe_steps=500)
@@ -172,7 +192,7 @@ def run_batched(C, args):
     seq_piece = seq_plain = seq_task = 0.0
     ctx = C.Context(0)
     for i, s in enumerate(shapes):
-        ctx.generate(s)
+        ctx.generate_device(s)
         ctx.set_option(C.OPT_FUSE_DSC, 0)
         seq_piece += ctx.run_loops(loops, policy=pol, mode=C.REDUCE_ATOMIC,
                                    per_piece=True)["elapsed_s"]
@@ -181,108 +201,195 @@ def run_batched(C, args):
         ctx.set_option(C.OPT_FUSE_DSC, 1)
         seq_task += ctx.run_loops(loops, policy=pol, mode=C.REDUCE_ATOMIC,
                                   use_graph=True)["elapsed_s"]
-        if i % 64 == 63:
+        if i % 128 == 127:
             log(f"[batched] sequential: {i + 1} circuits")
     ctx.close()
     wu = W_total * loops
+    return {"workload": "512 circuits, wires log-uniform in [2^10, 2^18], nodes = wires/4, "
+                        "pieces = ceil(nodes/4096), 95% in-piece, 10000 inner steps, "
+                        f"{loops} loops each, {policy_name.upper()} policy, ATOMIC reduction, "
+                        "hipGraph per loop",
+            "total_wires": W_total,
+            "autotuned": {"block_threads": best.block_threads, "persistent": best.persistent,
+                          "waves_per_simd": best.waves_per_simd},
+            "wire_updates_per_s": rep["wire_updates_per_s"], "elapsed_s": rep["elapsed_s"],
+            "sequential_per_piece": {"elapsed_s": seq_piece, "wire_updates_per_s": wu / seq_piece},
+            "sequential_plain": {"elapsed_s": seq_plain, "wire_updates_per_s": wu / seq_plain},
+            "sequential_fused_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
+            "speedup_vs_sequential_per_piece": seq_piece / rep["elapsed_s"],
+            "speedup_vs_sequential_plain": seq_plain / rep["elapsed_s"],
+            "speedup_vs_sequential_fused_graph": seq_task / rep["elapsed_s"]}
+
+
+def run_batched(C, args):
+    d = batched_measure(C, args.policy)
     out = {"metric": "batched wire-updates/s (512 circuits, one launch per task)",
-           "value": rep["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
+           "value": d["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
            "dtype": "f32", "data": "synthetic (load_circuit generator, seeds 1000+i)",
-           "config": {"workload": "512 circuits, wires log-uniform in [2^10, 2^18], nodes = wires/4, "
-                                  "pieces = ceil(nodes/4096), 95% in-piece, 10000 inner steps, "
-                                  f"5 loops each, {args.policy.upper()} policy, ATOMIC reduction, "
-                                  "hipGraph per loop",
-                      "total_wires": W_total,
-                      "autotuned": {"block_threads": best.block_threads,
-                                    "persistent": best.persistent,
-                                    "waves_per_simd": best.waves_per_simd}},
-           "batched_elapsed_s": rep["elapsed_s"],
-           "sequential_per_piece": {"elapsed_s": seq_piece, "wire_updates_per_s": wu / seq_piece},
-           "sequential_plain": {"elapsed_s": seq_plain, "wire_updates_per_s": wu / seq_plain},
-           "sequential_fused_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
-           "speedup_vs_sequential_per_piece": seq_piece / rep["elapsed_s"],
-           "speedup_vs_sequential_plain": seq_plain / rep["elapsed_s"],
-           "speedup_vs_sequential_fused_graph": seq_task / rep["elapsed_s"]}
+           "config": {"workload": d["workload"], "total_wires": d["total_wires"],
+                      "autotuned": d["autotuned"]},
+           **{k: v for k, v in d.items() if k not in ("workload", "total_wires", "autotuned")}}
     print(json.dumps(out))
+
+
+def plain_loop_times(C, pol, sizes, profile):
+    """Seconds per loop of one circuit alone with plain launches (one per task
+    over all its pieces, no fused epilogue, no graph), per wire count."""
+    out = []
+    ctx = C.Context(0)
+    ctx.set_option(C.OPT_FUSE_DSC, 0)
+    for W in sizes:
+        ctx.generate_device(C.shape_for_wires(int(W), 95, 77, profile))
+        ctx.run_loops(1, policy=pol, mode=C.REDUCE_ATOMIC)
+        out.append(ctx.run_loops(3, policy=pol, mode=C.REDUCE_ATOMIC)["elapsed_s"] / 3)
+    ctx.close()
+    return out
+
+
+def fifo(arrivals, service):
+    """One request at a time in arrival order: (latencies, end of the last)."""
+    free, lat = 0.0, []
+    for a, svc in zip(arrivals, service):
+        free = max(a, free) + svc
+        lat.append(free - a)
+    return lat, free
+
+
+def served_measure(C, policy_name, rate, seconds, cap_wires, timeout_s=0.0,
+                   compare_seconds=2.0):
+    """The served workload: a Poisson request stream served by continuous
+    batching, with p50/p99 latency; against one request at a time with plain
+    launches over the same trace."""
+    import numpy as np
+    budget = 200 * 10**9
+    pol = policy_of(C, policy_name)
+    trace = C.poisson_trace(rate, seconds, seed=5, timeout_s=timeout_s)
+    work = [r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops for r in trace]
+    log(f"[served] {len(trace)} requests over {seconds} s, {sum(work):.3g} wire-updates "
+        f"offered ({sum(work) / seconds:.3g}/s)")
+    s = C.Server(0, budget_bytes=budget, max_active_wires=cap_wires, policy=pol,
+                 mode=C.REDUCE_ATOMIC, gen_threads=12)
+    _, rep = s.run_trace(trace)
+    s.close()
+    log(f"[served] {rep}")
+    # One request at a time with plain launches, over the same trace: the
+    # per-loop time of a circuit alone, measured at 17 wire counts from 2^12
+    # to 2^20 and interpolated in log(wires), times each request's loops;
+    # queueing in arrival order.  GPU time only (no generation).
+    sizes = np.exp2(np.arange(12.0, 20.01, 0.5))
+    t_loop = np.array(plain_loop_times(C, pol, sizes, C.PROFILE_REFERENCE))
+    wires = np.array([r.gen.num_pieces * r.gen.wires_per_piece for r in trace], float)
+    svc = np.interp(np.log2(wires), np.log2(sizes), t_loop) * np.array([r.num_loops for r in trace])
+    lat, end = fifo([r.arrival_s for r in trace], svc)
+    span = end - (trace[0].arrival_s if trace else 0.0)
+    seq = {"wire_updates_per_s": sum(work) / span, "elapsed_s": span,
+           "latency_p50_s": float(np.percentile(lat, 50)),
+           "latency_p99_s": float(np.percentile(lat, 99)),
+           "per_loop_s_by_wires": {str(int(w)): t for w, t in zip(sizes, t_loop)}}
+    # The most any batching of these requests can gain over that: all of
+    # their work at the rate one circuit that fills the GPU runs at in the
+    # library's own loop (4,194,304 wires, fused epilogue, graph-captured).
+    ctx = C.Context(0)
+    ctx.generate_device(C.shape_for_wires(1 << 22, 95, 78, C.PROFILE_REFERENCE))
+    ctx.run_loops(1, policy=pol, mode=C.REDUCE_ATOMIC, use_graph=True)
+    full = ctx.run_loops(3, policy=pol, mode=C.REDUCE_ATOMIC, use_graph=True)
+    ctx.close()
+    r_full = full["wire_updates_per_s"]
+    bound = float(svc.sum() / (sum(work) / r_full))
+    log(f"[served] sequential plain (full trace): {seq['wire_updates_per_s']:.3g} wire-updates/s; "
+        f"batching bound {bound:.2f}x")
+    # Cross-check of that model on a short trace, measured directly.
+    short = C.poisson_trace(rate, compare_seconds, seed=6)
+    cmp = {}
+    s = C.Server(0, budget_bytes=budget, max_active_wires=cap_wires, policy=pol,
+                 mode=C.REDUCE_ATOMIC, gen_threads=12)
+    _, r2 = s.run_trace(short)
+    s.close()
+    cmp["continuous"] = {k: r2[k] for k in ("wire_updates_per_s", "elapsed_s", "latency_p50_s",
+                                            "latency_p99_s", "completed")}
+    ctx = C.Context(0)
+    ctx.set_option(C.OPT_FUSE_DSC, 0)
+    svc_short = []
+    for r in short:
+        ctx.generate_device(r.gen)
+        svc_short.append(ctx.run_loops(r.num_loops, policy=pol,
+                                       mode=C.REDUCE_ATOMIC)["elapsed_s"])
+    ctx.close()
+    w_short = np.array([r.gen.num_pieces * r.gen.wires_per_piece for r in short], float)
+    model_short = np.interp(np.log2(w_short), np.log2(sizes), t_loop) * \
+        np.array([r.num_loops for r in short])
+    lat2, end2 = fifo([r.arrival_s for r in short], svc_short)
+    cmp["sequential_plain_measured"] = {
+        "wire_updates_per_s": float((w_short * [r.num_loops for r in short]).sum()
+                                    / (end2 - short[0].arrival_s)),
+        "latency_p50_s": float(np.percentile(lat2, 50)),
+        "latency_p99_s": float(np.percentile(lat2, 99)),
+        "service_s": float(np.sum(svc_short)), "service_s_model": float(model_short.sum())}
+    log(f"[served] short trace: {cmp}")
+    return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s; wires log-uniform in "
+                        "[2^12, 2^20], nodes = wires/4, pieces = ceil(nodes/4096), 95% in-piece, "
+                        f"loops U[1,20], {policy_name.upper()} policy, ATOMIC reduction",
+            "hbm_budget_bytes": budget, "max_active_wires": cap_wires,
+            "requests": len(trace),
+            "wire_updates_per_s": rep["wire_updates_per_s"],
+            "offered_wire_updates_per_s": rep["offered_wire_updates_per_s"],
+            "latency_p50_s": rep["latency_p50_s"], "latency_p99_s": rep["latency_p99_s"],
+            "latency_mean_s": rep["latency_mean_s"], "latency_max_s": rep["latency_max_s"],
+            "completed": rep["completed"], "rejected": rep["rejected"],
+            "timed_out": rep["timed_out"], "iterations": rep["iterations"],
+            "max_resident": rep["max_resident"], "elapsed_s": rep["elapsed_s"],
+            "sequential_plain": seq,
+            "speedup_vs_sequential_plain": rep["wire_updates_per_s"] / seq["wire_updates_per_s"],
+            "full_gpu_rate": r_full,
+            "batching_bound_vs_sequential_plain": bound,
+            "short_trace_check": {"seconds": compare_seconds, **cmp}}
 
 
 def run_served(C, args):
-    budget = 200 * 10**9
-    pol = policy_of(C, args.policy)
-    trace = C.poisson_trace(args.serve_rate, args.serve_seconds, seed=5,
-                            timeout_s=args.serve_timeout)
-    offered = sum(r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops for r in trace)
-    log(f"[served] {len(trace)} requests over {args.serve_seconds} s, "
-        f"{offered:.3g} wire-updates offered ({offered / args.serve_seconds:.3g}/s)")
-    s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
-                 policy=pol, mode=C.REDUCE_ATOMIC, gen_threads=12)
-    res, rep = s.run_trace(trace)
-    s.close()
-    log(f"[served] {rep}")
-    # Continuous batching vs one request at a time on a short trace.
-    short = C.poisson_trace(args.serve_rate, args.compare_seconds, seed=6)
-    cmp = {}
-    for name, max_active in (("continuous", 0), ("sequential", 1)):
-        s = C.Server(0, budget_bytes=budget, max_active_wires=args.serve_cap_wires,
-                     max_active=max_active, policy=pol, mode=C.REDUCE_ATOMIC,
-                     gen_threads=12)
-        _, r2 = s.run_trace(short)
-        s.close()
-        cmp[name] = {k: r2[k] for k in ("wire_updates_per_s", "elapsed_s", "latency_p50_s",
-                                        "latency_p99_s", "completed")}
-        log(f"[served] {name}: {cmp[name]}")
-    # One request at a time in arrival order, in two launch styles, with
-    # service times measured on the GPU and queueing computed from the
-    # arrival times:
-    #  - plain: one launch per task over all of the request's pieces, no
-    #    fused epilogue, no graph;
-    #  - reference style: one launch per piece per task and a wait after each
-    #    (how the reference runtime executes GPU point tasks).
-    import numpy as np
-    for name, per_piece in (("sequential_plain", False), ("sequential_reference_style", True)):
-        ctx = C.Context(0)
-        ctx.set_option(C.OPT_FUSE_DSC, 0)
-        free_at, lat, wu = 0.0, [], 0.0
-        for r in short:
-            ctx.generate(r.gen)
-            svc = ctx.run_loops(r.num_loops, policy=pol, mode=C.REDUCE_ATOMIC,
-                                per_piece=per_piece)["elapsed_s"]
-            start = max(r.arrival_s, free_at)
-            free_at = start + svc
-            lat.append(free_at - r.arrival_s)
-            wu += r.gen.num_pieces * r.gen.wires_per_piece * r.num_loops
-        ctx.close()
-        span = free_at - (short[0].arrival_s if short else 0.0)
-        cmp[name] = {
-            "wire_updates_per_s": wu / span if span > 0 else 0.0, "elapsed_s": span,
-            "latency_p50_s": float(np.percentile(lat, 50)) if lat else 0.0,
-            "latency_p99_s": float(np.percentile(lat, 99)) if lat else 0.0,
-            "completed": len(lat)}
-        log(f"[served] {name}: {cmp[name]}")
+    d = served_measure(C, args.policy, args.serve_rate, args.serve_seconds,
+                       args.serve_cap_wires, args.serve_timeout, args.compare_seconds)
     out = {"metric": "served wire-updates/s with p50/p99 request latency",
-           "value": rep["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
+           "value": d["wire_updates_per_s"], "unit": "wire-updates/s", "n_gpus": 1,
            "dtype": "f32", "data": "synthetic (Poisson arrivals, load_circuit generator)",
-           "config": {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s; "
-                                  "wires log-uniform in [2^12, 2^20], nodes = wires/4, "
-                                  "pieces = ceil(nodes/4096), 95% in-piece, loops U[1,20], "
-                                  f"{args.policy.upper()} policy, ATOMIC reduction",
-                      "hbm_budget_bytes": budget, "max_active_wires": args.serve_cap_wires},
-           "latency_p50_s": rep["latency_p50_s"], "latency_p99_s": rep["latency_p99_s"],
-           "latency_mean_s": rep["latency_mean_s"], "latency_max_s": rep["latency_max_s"],
-           "offered_wire_updates_per_s": rep["offered_wire_updates_per_s"],
-           "completed": rep["completed"], "rejected": rep["rejected"],
-           "timed_out": rep["timed_out"], "admission_timeout_s": args.serve_timeout,
-           "iterations": rep["iterations"], "max_resident": rep["max_resident"],
-           "elapsed_s": rep["elapsed_s"],
-           "compare_short_trace": {
-               "seconds": args.compare_seconds, **cmp,
-               "speedup": cmp["continuous"]["wire_updates_per_s"]
-               / max(cmp["sequential"]["wire_updates_per_s"], 1e-30),
-               "speedup_vs_plain": cmp["continuous"]["wire_updates_per_s"]
-               / max(cmp["sequential_plain"]["wire_updates_per_s"], 1e-30),
-               "speedup_vs_reference_style": cmp["continuous"]["wire_updates_per_s"]
-               / max(cmp["sequential_reference_style"]["wire_updates_per_s"], 1e-30)}}
+           "config": {"workload": d["workload"], "hbm_budget_bytes": d["hbm_budget_bytes"],
+                      "max_active_wires": d["max_active_wires"]},
+           **{k: v for k, v in d.items()
+              if k not in ("workload", "hbm_budget_bytes", "max_active_wires")}}
     print(json.dumps(out))
+
+
+def fp64_errors(C, policy_name):
+    """Maximum relative node-voltage error against the FP64 oracle (the
+    relaxation's FP64 limit, circuit.cc:352-460 in double) per workload,
+    stable profile (in the reference profile the values are inf/NaN)."""
+    import numpy as np
+    from oracle import oracle as O
+    pol = policy_of(C, policy_name)
+    cases = [("4x1024x4096, 5 loops", (4, 1024, 4096, 95), 5, True),
+             ("4x1024x4096, 200 loops, 95% in-piece", (4, 1024, 4096, 95), 200, True),
+             ("4x1024x4096, 200 loops, 50% in-piece", (4, 1024, 4096, 50), 200, True),
+             ("64x16384x65536, 10 loops", (64, 16384, 65536, 95), 10, False)]
+    rows = {}
+    for name, (P, npp, wpp, pct), loops, all_variants in cases:
+        truth = O.run_f64_limit(O.generate(P, npp, wpp, pct, 12345, O.PROFILE_STABLE), loops,
+                                nthreads=host_threads())[0]["voltage"]
+        variants = [(f"{policy_name}+atomic", pol, C.REDUCE_ATOMIC),
+                    ("exact+csr", C.POLICY_EXACT, C.REDUCE_CSR)]
+        if all_variants:
+            variants.append(("exact+csr_kahan64", C.POLICY_EXACT, C.REDUCE_CSR_KAHAN64))
+        row = {}
+        for vname, pv, mode in variants:
+            with C.Context(0) as ctx:
+                ctx.generate_device(C.GenParams(P, npp, wpp, pct, 12345, C.PROFILE_STABLE))
+                ctx.run_loops(loops, policy=pv, mode=mode)
+                v = ctx.store_nodes()["voltage"].astype(np.float64)
+            fin = np.isfinite(v) & np.isfinite(truth)
+            row[vname] = float(np.max(np.abs(v[fin] - truth[fin]) /
+                                      np.maximum(np.abs(truth[fin]), 1e-30)))
+            row[vname + "_nonfinite"] = int((~np.isfinite(v)).sum())
+        rows[name] = row
+        log(f"[fp64] {name}: {row}")
+    return rows
 
 
 def run_upv_sweep(C, args):
@@ -533,7 +640,11 @@ def main():
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        # The reference's hand-coded single-GPU CUDA program (baseline.cu) on
+        # one Tesla M2090: 48 pieces x 2500 nodes x 16384 wires, 10 loops in
+        # 181.033 s (apps/circuit/keeneland_results/baseline_48_1_1_1.stdio),
+        # 43,441 wire-updates/s; a different circuit size, so context only.
+        "vs_baseline": value / BASELINE_WIRE_UPDATES_PER_S,
         "dtype": "f32",
         "data": f"synthetic (load_circuit generator, seed 12345, {args.profile} profile)",
         "config": {"workload": f"{P} pieces x {N // P} nodes x {W // P} wires/piece "
@@ -631,9 +742,26 @@ def main():
         ctx_s.close()
     out["detail"] = detail
     ctx.close()
+    # The rest of the metric: the batched and served workloads, and the FP64
+    # error of each workload.  A failure in one of them is recorded in its
+    # place and does not stop the line.
+    extra = [] if args.no_extra else [
+        ("batched", not args.skip_batched, lambda: batched_measure(C, args.policy)),
+        ("served", not args.skip_served,
+         lambda: served_measure(C, args.policy, args.serve_rate, args.serve_seconds,
+                                args.serve_cap_wires, args.serve_timeout, args.compare_seconds)),
+        ("fp64_error", not args.skip_fp64, lambda: fp64_errors(C, args.policy))]
+    for key, on, fn in extra:
+        if not on:
+            continue
+        try:
+            detail[key] = fn()
+        except Exception as e:  # recorded, the line is still printed
+            detail[key] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] {key} failed: {e}")
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        cb = cpu_baseline(C, params, args.cpu_seconds)
+        cb = cpu_baseline(C, params)
         out["cpu_baseline"] = {k: v for k, v in cb.items() if k != "seconds"}
         log(f"[bench] CPU baseline {cb['value']:.4g} wire-updates/s on {cb['cores']} threads")
     else:

@@ -435,6 +435,13 @@ class Context:
         _check(lib().ckt_ctx_pieces(self.h, _p(out)))
         return out
 
+    def store_nodes(self):
+        """The node records only (no wire download)."""
+        n, _, _ = self.sizes()
+        nodes = np.zeros(n, NODE)
+        _check(lib().ckt_ctx_store(self.h, _p(nodes), None))
+        return nodes
+
     def store(self):
         n, w, _ = self.sizes()
         nodes = np.zeros(n, NODE)
