@@ -492,6 +492,13 @@ int ckt_server_set_keep_state(ckt_server_t* s, int keep_state);
 int ckt_server_set_keep_limit(ckt_server_t* s, uint32_t max_wires);
 int ckt_server_fetch(ckt_server_t* s, uint32_t index, ckt_node_t* nodes, ckt_wire_t* wires);
 
+/* Debug builds (CKT_BOUNDS_CHECKS, the counterpart of the reference's
+ * POINTER_CHECKS accessors, runtime/lowlevel.h:818-832): the number of
+ * out-of-range indices the hot-path kernels found on `device` since the last
+ * call (those accesses were skipped), after waiting for the device.  Builds
+ * without the checks return CKT_ERR_UNSUPPORTED. */
+int ckt_debug_bounds_violations(int device, uint64_t* count);
+
 /* HBM calibration kernel: copies `bytes` from src to dst on the device with a
  * plain streaming kernel (used to scale profiler byte counters). */
 int ckt_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);

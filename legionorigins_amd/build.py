@@ -1,4 +1,6 @@
-"""Builds the gfx950 circuit library ``libckt.so`` in-tree with hipcc.
+"""Builds the gfx950 circuit library in-tree with hipcc: ``libckt.so``, and
+its bounds-checked (``libckt_checked.so``) and sanitized
+(``libckt_san.so``) variants.
 
 Every HIP and host source is compiled for gfx950 only, with
 ``-ffp-contract=off`` (explicit fmaf() is the only fused arithmetic) and with
@@ -34,27 +36,68 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), "--offload-arch=" + ARCH, *COMMON_FLAGS, "-x", "hip", "-c",
+# Build variants: (library, extra compile flags, extra link flags).
+#   release  the product;
+#   checked  with the kernels' bounds checks (CKT_BOUNDS_CHECKS);
+#   san      bounds checks, and the host code under AddressSanitizer and
+#            UndefinedBehaviorSanitizer (load it with the sanitizer runtime
+#            preloaded; the device code is not instrumented).
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-shared-libsan"]
+VARIANTS = {
+    "release": (LIB, [], []),
+    "checked": (os.path.join(HERE, "libckt_checked.so"), ["-DCKT_BOUNDS_CHECKS"], []),
+    "san": (os.path.join(HERE, "libckt_san.so"),
+            ["-DCKT_BOUNDS_CHECKS", "-g", "-fno-omit-frame-pointer", *SAN_HOST], SAN_HOST),
+}
+
+
+def sanitizer_runtime() -> str:
+    """The shared AddressSanitizer runtime the `san` variant links against
+    (to preload into a host process that loads it)."""
+    import glob
+    base = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    hits = sorted(glob.glob(os.path.join(base, "lib", "llvm", "lib", "clang", "*", "lib", "linux",
+                                         "libclang_rt.asan-x86_64.so")))
+    if not hits:
+        raise RuntimeError("libclang_rt.asan-x86_64.so not found under the ROCm LLVM tree")
+    return hits[-1]
+
+
+def _compile(job) -> str:
+    variant, src = job
+    flags = VARIANTS[variant][1]
+    out_dir = os.path.join(BUILD_DIR, variant)
+    obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), "--offload-arch=" + ARCH, *COMMON_FLAGS, *flags, "-x", "hip", "-c",
            os.path.join(CSRC, src), "-o", obj]
+    if variant == "san":
+        cmd = [c if c != "-O3" else "-O1" for c in cmd]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs,
+def _link(variant, objs):
+    lib, _, link_flags = VARIANTS[variant]
+    tmp = lib + ".tmp"
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", *link_flags, "-o", tmp, *objs,
            "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose: bool = False, variants=("release", "checked", "san")) -> str:
+    for v in variants:
+        os.makedirs(os.path.join(BUILD_DIR, v), exist_ok=True)
+    jobs = [(v, src) for v in variants for src in SOURCES]
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, jobs))
+    for v in variants:
+        _link(v, [o for (vv, _), o in zip(jobs, objs) if vv == v])
     # Host driver (region_main replacement) linked against the library.
     cmd = [hipcc(), "-O2", "-std=c++17", "-o", DRIVER + ".tmp", os.path.join(CSRC, "ckt_main.cpp"),
            "-L" + HERE, "-lckt", "-Wl,-rpath,$ORIGIN"]
@@ -63,10 +106,10 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print("built", LIB, "and", DRIVER)
+        print("built", ", ".join(VARIANTS[v][0] for v in variants), "and", DRIVER)
     return LIB
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, variants=tuple(sys.argv[1:]) or ("release", "checked", "san"))
     sys.exit(0)

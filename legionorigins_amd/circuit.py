@@ -15,7 +15,11 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libckt.so")
+# CKT_LIBRARY selects another build of the library: libckt_checked.so (kernel
+# bounds checks) or libckt_san.so (bounds checks, host code under
+# AddressSanitizer and UndefinedBehaviorSanitizer; load it with the sanitizer
+# runtime preloaded).
+LIB_PATH = os.environ.get("CKT_LIBRARY") or os.path.join(HERE, "libckt.so")
 
 # apps/circuit/circuit.h:42-44
 WIRE_SEGMENTS = 10
@@ -208,6 +212,7 @@ def _declare(L):
         "ckt_server_wait": ([vp, u32, ctypes.c_double, P(_RequestResult)], i32),
         "ckt_server_cancel": ([vp, u32], i32),
         "ckt_copy_kernel": ([vp, vp, sz, vp], i32),
+        "ckt_debug_bounds_violations": ([i32, P(u64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -368,6 +368,18 @@ int ckt_fold_apply_gpu(ckt_node_t* owner, const float* fold, uint32_t first, uin
   return finish(launch_fold_apply(owner, fold, first, count, s), s, "fold_apply");
 }
 
+int ckt_debug_bounds_violations(int device, uint64_t* count) {
+  if (!count) return fail(CKT_ERR_INVALID_ARG, "ckt_debug_bounds_violations: null count");
+  *count = 0;
+  CKT_HIP(hipSetDevice(device));
+  CKT_HIP(hipDeviceSynchronize());
+  const hipError_t e = read_bounds_violations(count);
+  if (e == hipErrorNotSupported)
+    return fail(CKT_ERR_UNSUPPORTED, "this build has no bounds checks (CKT_BOUNDS_CHECKS)");
+  if (e != hipSuccess) return hip_fail(e, "bounds-check counter");
+  return CKT_OK;
+}
+
 int ckt_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
   if (!dst || !src) return fail(CKT_ERR_INVALID_ARG, "ckt_copy_kernel: null pointer");
   if (bytes % 16) return fail(CKT_ERR_INVALID_ARG, "ckt_copy_kernel: size must be a multiple of 16");

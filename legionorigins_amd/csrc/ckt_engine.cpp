@@ -238,6 +238,7 @@ Engine::~Engine() {
   if (d_table_) (void)hipFree(d_table_);
   if (d_counters_) (void)hipFree(d_counters_);
   if (d_bad_) (void)hipFree(d_bad_);
+  if (d_partial_) (void)hipFree(d_partial_);
   if (devgen_scratch_) (void)hipFree(devgen_scratch_);
   for (const PinnedChunk& c : staging_) (void)hipHostFree(c.p);
   for (auto& p : pending_) {
@@ -617,9 +618,15 @@ int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, De
     return CKT_OK;
   }
   if (kind == TK_PIECES) {
+    // `block` is the number of wire subranges per piece; each piece's rows of
+    // partial sums follow the previous piece's.
     std::vector<PieceTile> t;
     t.reserve(refs.size());
-    for (const PieceRef& r : refs) t.push_back(PieceTile{r.slot, r.piece});
+    uint64_t off = 0;
+    for (const PieceRef& r : refs) {
+      t.push_back(PieceTile{r.slot, r.piece, (uint32_t)off, 0});
+      off += (uint64_t)std::max(block, 1) * slots_[r.slot]->pieces[r.piece].num_nodes;
+    }
     return upload_buf(key, t.data(), t.size() * sizeof(PieceTile), (uint32_t)t.size(), out);
   }
   std::vector<Tile> t;
@@ -751,7 +758,9 @@ int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
   if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
   if (mode == CKT_REDUCE_ATOMIC && !fuse_dsc_) {
     if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
-    if ((st = tiles_for(TK_PIECES, refs, 0, &t))) return st;
+    LdsPlan plan;
+    if ((st = lds_plan(refs, &plan))) return st;
+    if (plan.lds && (st = tiles_for(TK_PIECES, refs, (int)plan.splits, &t))) return st;
   }
   if (mode != CKT_REDUCE_ATOMIC) {
     if ((st = csr_nodes_ok(refs))) return st;
@@ -793,6 +802,44 @@ int Engine::csr_nodes_ok(const std::vector<PieceRef>& refs) const {
   return CKT_OK;
 }
 
+int Engine::lds_plan(const std::vector<PieceRef>& refs, LdsPlan* plan) {
+  LdsPlan p;
+  p.lds = dsc_atomic_impl_ != 1 && !refs.empty();
+  uint32_t min_wires = UINT32_MAX;
+  uint64_t nodes = 0;
+  for (const PieceRef& r : refs) {
+    const Circuit* c = slots_[r.slot];
+    const ckt_piece_t& pc = c->pieces[r.piece];
+    p.lds = p.lds && c->lds_ok;
+    p.max_nodes = std::max(p.max_nodes, pc.num_nodes);
+    min_wires = std::min(min_wires, pc.num_wires);
+    nodes += pc.num_nodes;
+  }
+  p.lds_bytes = dsc_lds_bytes(p.max_nodes);
+  p.lds = p.lds && p.lds_bytes <= lds_limit_;
+  if (p.lds) {
+    // Two workgroups per CU, with no subrange under 2048 wires and at most 16
+    // per piece.
+    const uint64_t want = ((uint64_t)2 * num_cus_ + refs.size() - 1) / refs.size();
+    const uint64_t by_size = std::max<uint32_t>(1, min_wires / 2048);
+    p.splits = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, by_size, 16}));
+    if ((uint64_t)p.splits * nodes > 0xffffffffull) p.splits = 1;
+    // Rows of partial sums even with one subrange: the second kernel's plain
+    // read-add-write of every node replaces an atomic per shared node.
+    p.partial_floats = (uint64_t)p.splits * nodes <= 0xffffffffull ? (uint64_t)p.splits * nodes : 0;
+    if (p.partial_floats > partial_cap_) {
+      CKT_HIP(hipStreamSynchronize(stream_));
+      if (d_partial_) CKT_HIP(hipFree(d_partial_));
+      d_partial_ = nullptr;
+      partial_cap_ = 0;
+      CKT_HIP(hipMalloc((void**)&d_partial_, p.partial_floats * sizeof(float)));
+      partial_cap_ = p.partial_floats;
+    }
+  }
+  *plan = p;
+  return CKT_OK;
+}
+
 int Engine::dsc(const std::vector<PieceRef>& refs, int mode) {
   int st;
   if (mode < CKT_REDUCE_ATOMIC || mode > CKT_REDUCE_CSR_KAHAN64)
@@ -802,20 +849,15 @@ int Engine::dsc(const std::vector<PieceRef>& refs, int mode) {
   hipError_t e;
   hipEvent_t a;
   if (mode == CKT_REDUCE_ATOMIC) {
-    size_t lds_bytes = 0;
-    bool lds = dsc_atomic_impl_ != 1;
-    for (const PieceRef& r : refs) {
-      const Circuit* c = slots_[r.slot];
-      lds = lds && c->lds_ok;
-      lds_bytes = std::max<size_t>(lds_bytes, (size_t)c->pieces[r.piece].num_nodes * 4);
-    }
-    lds = lds && lds_bytes <= lds_limit_;
-    if (dsc_atomic_impl_ == 2 && !lds)
+    LdsPlan plan;
+    if ((st = lds_plan(refs, &plan))) return st;
+    if (dsc_atomic_impl_ == 2 && !plan.lds)
       return fail(CKT_ERR_UNSUPPORTED, "LDS reduction: a piece's node range does not fit in LDS");
-    if (lds) {
-      if ((st = tiles_for(TK_PIECES, refs, 0, &t))) return st;
+    if (plan.lds) {
+      if ((st = tiles_for(TK_PIECES, refs, (int)plan.splits, &t))) return st;
       if ((st = timed_begin(CKT_DISTRIBUTE_CHARGE, &a))) return st;
-      e = launch_dsc_lds(d_table_, (const PieceTile*)t.p, t.n, std::max<size_t>(lds_bytes, 4), stream_);
+      e = launch_dsc_lds(d_table_, (const PieceTile*)t.p, t.n, plan.splits, plan.max_nodes,
+                         plan.partial_floats ? d_partial_ : nullptr, plan.lds_bytes, stream_);
     } else {
       if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
       if ((st = timed_begin(CKT_DISTRIBUTE_CHARGE, &a))) return st;

@@ -183,6 +183,16 @@ class Engine {
   int timed_begin(int task, hipEvent_t* a);
   int timed_end(int task, hipEvent_t a);
   int csr_nodes_ok(const std::vector<PieceRef>& refs) const;
+  // The LDS-privatised ATOMIC reduction over `refs`: whether it applies, its
+  // LDS bytes, the subranges per piece and the floats of partial sums it
+  // needs (allocated here, before any launch or capture).
+  struct LdsPlan {
+    bool lds = false;
+    size_t lds_bytes = 0;
+    uint32_t splits = 1, max_nodes = 0;
+    uint64_t partial_floats = 0;
+  };
+  int lds_plan(const std::vector<PieceRef>& refs, LdsPlan* plan);
 
   int device_ = -1;
   hipStream_t stream_ = nullptr;
@@ -200,6 +210,8 @@ class Engine {
   std::unordered_map<uint64_t, hipGraphExec_t> graphs_;
   uint32_t* d_counters_ = nullptr;
   unsigned long long* d_bad_ = nullptr;
+  float* d_partial_ = nullptr;  // split LDS reduction's partial sums (grow-only)
+  uint64_t partial_cap_ = 0;    // floats
   int steps_ = CKT_STEPS;
   int cnc_block_ = 256;
   int persistent_ = -1;      // -1: chosen per policy (launch_shape)

@@ -51,6 +51,8 @@ struct Tile {
 struct PieceTile {
   uint32_t circuit;
   uint32_t piece;
+  uint32_t partial;  // split LDS reduction: first float of the piece's rows in `partial`
+  uint32_t pad;
 };
 
 // Host launchers (defined in ckt_kernels.hip).  All are asynchronous on `s`.
@@ -63,8 +65,16 @@ hipError_t launch_cnc_persistent(int policy, bool fuse_dsc, const DevCircuit* ci
                                  uint32_t* counters, int steps, hipStream_t s);
 hipError_t launch_dsc_atomic(const DevCircuit* circuits, const Tile* tiles, uint32_t ntiles,
                              int block, hipStream_t s);
+// Dynamic LDS bytes of the LDS-privatised reduction for pieces of up to
+// max_nodes nodes.
+size_t dsc_lds_bytes(uint32_t max_nodes);
+// LDS-privatised ATOMIC reduction over n piece tiles, each piece's wires in
+// `splits` subranges (splits > 1 needs `partial`, splits x the tiles' node
+// counts floats at each tile's `partial` offset, and max_nodes, the largest
+// tile's node count).
 hipError_t launch_dsc_lds(const DevCircuit* circuits, const PieceTile* ptiles, uint32_t n,
-                          size_t lds_bytes, hipStream_t s);
+                          uint32_t splits, uint32_t max_nodes, float* partial, size_t lds_bytes,
+                          hipStream_t s);
 hipError_t launch_dsc_csr(int kahan, const DevCircuit* circuits, const Tile* node_tiles,
                           uint32_t ntiles, int block, hipStream_t s);
 // nontemporal: streaming accesses for node arrays larger than the last-level
@@ -103,6 +113,7 @@ hipError_t launch_fold_init(float* fold, uint32_t first, uint32_t count, hipStre
 hipError_t launch_fold_apply(ckt_node_t* owner, const float* fold, uint32_t first, uint32_t count,
                              hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+hipError_t read_bounds_violations(uint64_t* count);
 
 // Error reporting (thread-local last error).
 int fail(int status, const std::string& msg);

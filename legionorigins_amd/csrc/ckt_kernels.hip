@@ -11,6 +11,7 @@
 // generated (-packed-fp32-ops).
 #include <hip/hip_runtime.h>
 
+#include "ckt_index.h"
 #include "ckt_internal.h"
 
 namespace ckt {
@@ -542,6 +543,8 @@ __device__ __forceinline__ void cnc_one(const DevCircuit& c, uint32_t w, int ste
   const size_t ws = c.ws;
   gfloat* cur = (gfloat*)c.cur;
   gfloat* vol = (gfloat*)c.vol;
+  if (!CKT_IN_BOUNDS(w < c.num_wires && c.in_ptr[w] < c.num_nodes && c.out_ptr[w] < c.num_nodes))
+    return;
   float I[kSeg], V[kSeg - 1];
 #pragma unroll
   for (int s = 0; s < kSeg; s++) I[s] = cur[s * ws + w];
@@ -610,6 +613,8 @@ __global__ __launch_bounds__(256) void dsc_atomic_kernel(const DevCircuit* __res
   if (threadIdx.x >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
   const uint32_t w = t.first + threadIdx.x;
+  if (!CKT_IN_BOUNDS(w < c.num_wires && c.in_ptr[w] < c.num_nodes && c.out_ptr[w] < c.num_nodes))
+    return;
   const float dt = CKT_DT_F;
   const float q_in = -dt * c.cur[w];
   const float q_out = dt * c.cur[(size_t)(kSeg - 1) * c.ws + w];
@@ -617,55 +622,127 @@ __global__ __launch_bounds__(256) void dsc_atomic_kernel(const DevCircuit* __res
   unsafeAtomicAdd(&c.charge[c.out_ptr[w]], q_out);
 }
 
-// One workgroup per piece: contributions to the piece's own node range are
-// summed in LDS, everything else goes straight to HBM with float atomics.
-// Private nodes receive contributions only from their own piece's wires, so
-// they are flushed with a plain read-add-write; shared nodes with atomics.
+#ifdef CKT_BOUNDS_CHECKS
+static __device__ unsigned long long g_bounds_violations;
+__device__ bool bounds_violation() {
+  atomicAdd(&g_bounds_violations, 1ull);
+  return false;
+}
+#endif
+
+// Device-scope float add to global memory, result unused.
+__device__ __forceinline__ void global_add(gfloat* p, float v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS-privatised reduction.  Workgroup b takes wire subrange b % splits of
+// piece tile b / splits (the piece's wires cut into `splits` ranges of whole
+// 1024-wire blocks).  Contributions to the piece's own node range are summed
+// in LDS, everything else goes straight to HBM with float atomics.  Each
+// workgroup writes its sums to its own row of `partial` (plain stores) and
+// dsc_lds_sum_kernel adds the rows to the nodes after every atomic of this
+// launch has landed.  Without a partial buffer (one subrange per piece) the
+// workgroup adds its sums to the piece's nodes itself: private nodes receive
+// contributions only from their own piece's wires, so they take a plain
+// read-add-write, shared nodes (which other pieces' workgroups reach)
+// atomics.  Each lane loads four wires' contributions before it adds any, so
+// that 16 loads are in flight per lane.  (LDS float atomics, ds_add_f32, take
+// about a lane per clock per CU and bound the kernel; integer LDS atomics
+// and plain LDS accesses cost far less, but a counting sort of each chunk's
+// contributions built on them, with its scan and barriers and one
+// 160 KB workgroup per CU, ran at half this speed.)
+template <bool SPLIT>
 __global__ __launch_bounds__(1024) void dsc_lds_kernel(const DevCircuit* __restrict__ circuits,
-                                                       const PieceTile* __restrict__ ptiles) {
+                                                       const PieceTile* __restrict__ ptiles,
+                                                       uint32_t splits, float* __restrict__ partial) {
   extern __shared__ float acc[];
-  const PieceTile pt = ptiles[blockIdx.x];
-  const DevCircuit& c = circuits[pt.circuit];
-  const ckt_piece_t p = c.pieces[pt.piece];
-  for (uint32_t i = threadIdx.x; i < p.num_nodes; i += blockDim.x) acc[i] = 0.f;
+  const uint32_t si = blockIdx.x % splits;
+  const PieceTile pt = ptiles[blockIdx.x / splits];
+  const ckt_piece_t p = circuits[pt.circuit].pieces[pt.piece];
+  // The circuit's arrays, read once: the atomics below write memory the
+  // compiler cannot tell apart from the descriptor table, so fields read
+  // through it would be reloaded after every atomic.
+  const uint32_t ws = circuits[pt.circuit].ws;
+  const uint32_t num_wires = circuits[pt.circuit].num_wires;
+  const uint32_t num_nodes = circuits[pt.circuit].num_nodes;
+  if (!CKT_IN_BOUNDS((uint64_t)p.first_node + p.num_nodes <= num_nodes)) return;
+  const gfloat* cur0 = (const gfloat*)circuits[pt.circuit].cur;
+  const gfloat* cur9 = cur0 + (size_t)(kSeg - 1) * ws;
+  const gcu32* in_ptr = (const gcu32*)circuits[pt.circuit].in_ptr;
+  const gcu32* out_ptr = (const gcu32*)circuits[pt.circuit].out_ptr;
+  gfloat* charge = (gfloat*)circuits[pt.circuit].charge;
+  const uint8_t* locator = circuits[pt.circuit].locator;
+  uint32_t k0, k1;
+  lds_subrange(p.num_wires, splits, si, &k0, &k1);
+  const uint32_t npp = p.num_nodes;
+  for (uint32_t i = threadIdx.x; i < npp; i += blockDim.x) acc[i] = 0.f;
   __syncthreads();
   const float dt = CKT_DT_F;
-  const size_t off9 = (size_t)(kSeg - 1) * c.ws;
-  for (uint32_t k = threadIdx.x; k < p.num_wires; k += blockDim.x) {
-    const uint32_t w = p.first_wire + k;
-    const float q_in = -dt * c.cur[w];
-    const float q_out = dt * c.cur[off9 + w];
-    const uint32_t a = c.in_ptr[w], b = c.out_ptr[w];
-    const uint32_t la = a - p.first_node, lb = b - p.first_node;
-    if (la < p.num_nodes) atomicAdd(&acc[la], q_in);
-    else unsafeAtomicAdd(&c.charge[a], q_in);
-    if (lb < p.num_nodes) atomicAdd(&acc[lb], q_out);
-    else unsafeAtomicAdd(&c.charge[b], q_out);
+  const uint32_t stride = blockDim.x;
+  constexpr int kU = 4;
+  for (uint32_t base = k0 + threadIdx.x; base < k1; base += kU * stride) {
+    float q[2 * kU];
+    uint32_t node[2 * kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const uint32_t k = base + u * stride;
+      uint32_t w = p.first_wire + (k < k1 ? k : k0);
+      if (!CKT_IN_BOUNDS(w < num_wires)) w = 0;
+      q[2 * u] = -dt * cur0[w];
+      q[2 * u + 1] = dt * cur9[w];
+      node[2 * u] = in_ptr[w];
+      node[2 * u + 1] = out_ptr[w];
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * kU; j++) {
+      if (base + (j >> 1) * stride >= k1) break;
+      if (!CKT_IN_BOUNDS(node[j] < num_nodes)) continue;
+      const uint32_t l = node[j] - p.first_node;
+      if (l < npp) atomicAdd(&acc[l], q[j]);
+      else global_add(&charge[node[j]], q[j]);
+    }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < p.num_nodes; i += blockDim.x) {
-    const uint32_t n = p.first_node + i;
-    const float v = acc[i];
-    if (c.locator[n]) c.charge[n] = c.charge[n] + v;
-    else if (v != 0.f) unsafeAtomicAdd(&c.charge[n], v);
+  if constexpr (SPLIT) {
+    float* row = partial + pt.partial + (size_t)si * npp;
+    for (uint32_t i = threadIdx.x; i < npp; i += blockDim.x) row[i] = acc[i];
+  } else {
+    for (uint32_t i = threadIdx.x; i < npp; i += blockDim.x) {
+      const uint32_t n = p.first_node + i;
+      const float v = acc[i];
+      if (locator[n]) charge[n] = charge[n] + v;
+      else if (v != 0.f) global_add(&charge[n], v);
+    }
   }
+}
+
+// The rows of the split reduction added to the nodes: node i of piece tile
+// blockIdx.y, in subrange order.  Every atomic of the first kernel (other
+// pieces' contributions to this piece's shared nodes) has landed, and piece
+// node ranges are disjoint, so each node has one writer here.
+__global__ __launch_bounds__(256) void dsc_lds_sum_kernel(const DevCircuit* __restrict__ circuits,
+                                                          const PieceTile* __restrict__ ptiles,
+                                                          uint32_t splits,
+                                                          const float* __restrict__ partial) {
+  const PieceTile pt = ptiles[blockIdx.y];
+  const DevCircuit& c = circuits[pt.circuit];
+  const ckt_piece_t p = c.pieces[pt.piece];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= p.num_nodes) return;
+  if (!CKT_IN_BOUNDS(p.first_node + i < c.num_nodes)) return;
+  const float* row = partial + pt.partial + i;
+  float v = 0.f;
+  for (uint32_t s = 0; s < splits; s++) v += row[(size_t)s * p.num_nodes];
+  const uint32_t n = p.first_node + i;
+  c.charge[n] = c.charge[n] + v;
 }
 
 // One lane per node: the node's contributions in (wire, in-before-out) order,
 // added one at a time to its charge -- the order in which the reference's
 // CPU tasks apply them when run one piece after another.
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
-// XCD b % 8), and each XCD has its own L2.  xcd_tile gives the workgroups of
-// one XCD a contiguous range of the n tiles, so neighbouring node tiles --
+// Node tiles in XCD-contiguous order (xcd_tile): neighbouring node tiles --
 // whose wires, and so whose gathered currents, lie in the same piece -- share
-// an L2 instead of being spread over all eight.  A bijection on [0, n).
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
-  const uint32_t q = n / kXcds, r = n % kXcds;
-  const uint32_t x = b % kXcds, k = b / kXcds;
-  return x * q + (x < r ? x : r) + k;
-}
-
+// an L2.
 template <bool KAHAN>
 __global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restrict__ circuits,
                                                       const Tile* __restrict__ tiles) {
@@ -673,15 +750,18 @@ __global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restri
   if (threadIdx.x >= t.count) return;
   const DevCircuit& c = circuits[t.circuit];
   const uint32_t n = t.first + threadIdx.x;
+  if (!CKT_IN_BOUNDS(n < c.num_nodes)) return;
   const float dt = CKT_DT_F;
   const float* cur0 = c.cur;
   const float* cur9 = c.cur + (size_t)(kSeg - 1) * c.ws;
   const uint32_t k0 = c.csr_row[n], k1 = c.csr_row[n + 1];
+  if (!CKT_IN_BOUNDS(k0 <= k1 && k1 <= 2ull * c.num_wires)) return;
   if (!KAHAN || c.locator[n]) {
     float acc = c.charge[n];
     for (uint32_t k = k0; k < k1; k++) {
       const uint32_t e = c.csr_ent[k];
       const uint32_t w = e >> 1;
+      if (!CKT_IN_BOUNDS(w < c.num_wires)) continue;
       const float v = (e & 1u) ? dt * cur9[w] : -dt * cur0[w];
       acc += v;
     }
@@ -691,6 +771,7 @@ __global__ __launch_bounds__(256) void dsc_csr_kernel(const DevCircuit* __restri
     for (uint32_t k = k0; k < k1; k++) {
       const uint32_t e = c.csr_ent[k];
       const uint32_t w = e >> 1;
+      if (!CKT_IN_BOUNDS(w < c.num_wires)) continue;
       const float v = (e & 1u) ? dt * cur9[w] : -dt * cur0[w];
       const double y = (double)v - comp;
       const double tt = s + y;
@@ -742,9 +823,10 @@ __global__ __launch_bounds__(256) void upv_kernel(const DevCircuit* __restrict__
   gfloat* ncap = (gfloat*)c.ncap;
   gfloat* leak = (gfloat*)c.leak;
   const bool aligned = (t.first & 3u) == 0;
+  if (!CKT_IN_BOUNDS((uint64_t)t.first + t.count <= c.num_nodes && t.count <= 2048u)) return;
 #pragma unroll
   for (uint32_t g = 0; g < 2; g++) {
-    const uint32_t k0 = 4u * (g * 256u + threadIdx.x);
+    const uint32_t k0 = upv_group_first(g, threadIdx.x);
     if (k0 >= t.count) continue;
     const uint32_t n = t.first + k0;
     if (aligned && k0 + 4 <= t.count) {
@@ -830,16 +912,7 @@ __device__ __forceinline__ const ckt_node_t* pick_instance(int32_t loc, const ck
 // covering 256 consecutive elements (wires or nodes) of it.
 __device__ __forceinline__ uint32_t piece_of_block(const PieceIndex& ix, uint32_t b,
                                                    uint32_t* elem) {
-  // The last piece p with first_block[p] <= b (pieces without elements own
-  // no workgroup and are passed over).
-  uint32_t lo = 0, hi = ix.num_pieces;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (ix.first_block[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  *elem = (b - ix.first_block[lo]) * 256u + threadIdx.x;
-  return lo;
+  return piece_of_block(ix.first_block, ix.num_pieces, b, threadIdx.x, elem);
 }
 
 template <int POLICY>
@@ -1019,18 +1092,41 @@ hipError_t launch_dsc_atomic(const DevCircuit* c, const Tile* t, uint32_t n, int
   return hipGetLastError();
 }
 
-hipError_t launch_dsc_lds(const DevCircuit* c, const PieceTile* pt, uint32_t n, size_t lds_bytes,
-                          hipStream_t s) {
-  if (n == 0) return hipSuccess;
+// Bytes of dynamic LDS the reduction needs for pieces of up to max_nodes
+// nodes.
+size_t dsc_lds_bytes(uint32_t max_nodes) {
+  const size_t b = (size_t)max_nodes * 4;
+  return b < 256 ? 256 : b;
+}
+
+template <bool SPLIT>
+static hipError_t dsc_lds_go(const DevCircuit* c, const PieceTile* pt, uint32_t grid,
+                             uint32_t splits, float* partial, size_t lds_bytes, hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
     // Allows dynamic LDS beyond the default limit where the runtime needs it.
-    (void)hipFuncSetAttribute((const void*)dsc_lds_kernel,
+    (void)hipFuncSetAttribute((const void*)dsc_lds_kernel<SPLIT>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
     attr_set = true;
   }
-  hipLaunchKernelGGL(dsc_lds_kernel, dim3(n), dim3(1024), lds_bytes, s, c, pt);
+  hipLaunchKernelGGL(dsc_lds_kernel<SPLIT>, dim3(grid), dim3(1024), lds_bytes, s, c, pt, splits,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_dsc_lds(const DevCircuit* c, const PieceTile* pt, uint32_t n, uint32_t splits,
+                          uint32_t max_nodes, float* partial, size_t lds_bytes, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (splits < 1 || (splits > 1 && !partial)) return hipErrorInvalidValue;
+  if (lds_bytes < dsc_lds_bytes(max_nodes)) return hipErrorInvalidValue;
+  const uint64_t grid = (uint64_t)n * splits;
+  if (grid > 0x7fffffffu) return hipErrorInvalidValue;
+  hipError_t e = partial ? dsc_lds_go<true>(c, pt, (uint32_t)grid, splits, partial, lds_bytes, s)
+                         : dsc_lds_go<false>(c, pt, (uint32_t)grid, 1u, nullptr, lds_bytes, s);
+  if (e != hipSuccess || !partial || max_nodes == 0) return e;
+  hipLaunchKernelGGL(dsc_lds_sum_kernel, dim3(blocks_for(max_nodes, 256), n), dim3(256), 0, s, c,
+                     pt, splits, (const float*)partial);
   return hipGetLastError();
 }
 
@@ -1131,6 +1227,23 @@ hipError_t launch_fold_apply(ckt_node_t* owner, const float* fold, uint32_t firs
   hipLaunchKernelGGL(fold_apply_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, owner, fold,
                      first, count);
   return hipGetLastError();
+}
+
+// The bounds-check counter of this device (read and cleared), or
+// hipErrorNotSupported in builds without CKT_BOUNDS_CHECKS.
+hipError_t read_bounds_violations(uint64_t* count) {
+#ifdef CKT_BOUNDS_CHECKS
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpyFromSymbol(&h, HIP_SYMBOL(g_bounds_violations), sizeof(h));
+  if (e != hipSuccess) return e;
+  const unsigned long long zero = 0;
+  e = hipMemcpyToSymbol(HIP_SYMBOL(g_bounds_violations), &zero, sizeof(zero));
+  *count = h;
+  return e;
+#else
+  *count = 0;
+  return hipErrorNotSupported;
+#endif
 }
 
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {

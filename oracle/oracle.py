@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcircuit_oracle.so")
+# ORACLE_LIBRARY selects another build (libcircuit_oracle_san.so: under
+# AddressSanitizer and UndefinedBehaviorSanitizer, `make -C oracle san`).
+LIB_PATH = os.environ.get("ORACLE_LIBRARY") or os.path.join(HERE, "libcircuit_oracle.so")
 
 SEG = 10
 STEPS = 10000
@@ -38,7 +40,7 @@ _lib = None
 
 
 def build() -> None:
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "all", "san"], check=True)
 
 
 def lib():

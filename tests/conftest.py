@@ -28,3 +28,18 @@ def C():
 
 def pytest_report_header(config):
     return [f"threads for the oracle: {min(16, os.cpu_count() or 1)}"]
+
+
+@pytest.fixture(autouse=True)
+def _no_bounds_violations(request):
+    """In a run with the bounds-checked library, every GPU test must leave the
+    device's count of out-of-range kernel indices at 0."""
+    yield
+    if os.environ.get("CKT_CHECKED_RUN") != "1" or request.node.get_closest_marker("gpu") is None:
+        return
+    import ctypes
+    from legionorigins_amd import circuit
+    n = ctypes.c_uint64()
+    st = circuit.lib().ckt_debug_bounds_violations(0, ctypes.byref(n))
+    assert st == 0, circuit.lib().ckt_last_error()
+    assert n.value == 0, f"{n.value} out-of-range kernel indices"

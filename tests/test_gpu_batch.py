@@ -105,21 +105,29 @@ def test_autotune_restores_state(C):
     b.close()
 
 
-def test_atomic_implementations_agree(C):
+@pytest.mark.parametrize("shape", [(8, 2048, 8192), (3, 5000, 20000), (600, 64, 256)],
+                         ids=["8-pieces", "3-pieces-ragged-subranges", "600-pieces"])
+def test_atomic_implementations_agree(C, shape):
+    # Global atomics, the LDS-privatised reduction (its pieces split into
+    # 4 and 9 wire subranges with partial sums in the first two shapes, one
+    # workgroup per piece in the third) and the atomics fused into
+    # calc_new_currents, against the deterministic CSR reduction.
+    P, npp, wpp = shape
     outs = []
     for fuse, impl in ((0, 1), (0, 2), (1, 0)):
         with C.Context(0) as ctx:
-            ctx.generate(C.GenParams(8, 2048, 8192, 80, 9, C.PROFILE_STABLE))
+            ctx.generate(C.GenParams(P, npp, wpp, 80, 9, C.PROFILE_STABLE))
             ctx.set_option(C.OPT_FUSE_DSC, fuse)
             ctx.set_option(C.OPT_DSC_ATOMIC_IMPL, impl)
             ctx.run_loops(2, policy=C.POLICY_FAST, mode=C.REDUCE_ATOMIC)
             outs.append(ctx.store()[0])
     with C.Context(0) as ctx:
-        ctx.generate(C.GenParams(8, 2048, 8192, 80, 9, C.PROFILE_STABLE))
+        ctx.generate(C.GenParams(P, npp, wpp, 80, 9, C.PROFILE_STABLE))
         ctx.run_loops(2, policy=C.POLICY_FAST, mode=C.REDUCE_CSR)
         csr = ctx.store()[0]
     for o in outs:
         assert max_rel_err(o["voltage"], csr["voltage"]) <= TOL
+        assert not o["charge"].any()
 
 
 def _c4_shapes(C, profile):
