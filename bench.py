@@ -85,6 +85,11 @@ def parse():
                     help="update_voltages HBM bandwidth at growing node counts")
     ap.add_argument("--dsc-sweep", action="store_true",
                     help="distribute_charge bandwidth per reduction mode at growing sizes")
+    ap.add_argument("--dsc-pieces", default="64,256,1024",
+                    help="dsc sweep: circuit sizes in pieces of 16384 nodes x 65536 wires")
+    ap.add_argument("--dsc-modes", default=None,
+                    help="dsc sweep: comma-separated subset of atomic_global,atomic_lds,csr,kahan64")
+    ap.add_argument("--dsc-reps", type=int, default=20, help="dsc sweep: timed launches per mode")
     ap.add_argument("--accessor", action="store_true",
                     help="the drop-in accessor form at 64x16384x65536: per-piece synchronous calls on "
                          "reference-layout instances")
@@ -176,14 +181,26 @@ def batched_measure(C, policy_name, loops=5):
     b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, host_threads()))
     log(f"[batched] {len(shapes)} circuits, {W_total} wires, added in "
         f"{time.perf_counter() - t0:.1f} s")
-    best, probe_ms = b.autotune(pol, probe_steps=500)
-    log(f"[batched] autotune: block={best.block_threads} persistent={best.persistent} "
-        f"waves/SIMD={best.waves_per_simd} ({probe_ms:.2f} ms per 500-step probe)")
+    # The library's default schedule (256-thread persistent workgroups, 4
+    # resident waves per SIMD, one tile per 256 wires of a piece, pieces in
+    # order) against the autotuned one, as probes and as whole runs.
+    default = C.Batch.config(policy=pol, mode=C.REDUCE_ATOMIC, block_threads=256, persistent=1,
+                             waves_per_simd=4, wires_per_lane=1, tile_span=0, tile_order=0)
+    default_probe_ms = b.probe(default, probe_steps=2000)
+    best, _ = b.autotune(pol, probe_steps=2000)
+    best_probe_ms = b.probe(best, probe_steps=2000)
+    log(f"[batched] autotune: {best.describe()} ({best_probe_ms:.2f} ms per 2000-step probe, "
+        f"default schedule {default_probe_ms:.2f} ms)")
     best.policy, best.reduce_mode, best.use_graph = pol, C.REDUCE_ATOMIC, 1
     b.set_timing(True)
     rep = b.run(best)
     b.close()
     log(f"[batched] {rep}")
+    b = C.Batch(0)
+    b.add_generated_many(shapes, [loops] * len(shapes), threads=min(16, host_threads()))
+    rep_default = b.run(default)
+    b.close()
+    log(f"[batched] default schedule: {rep_default}")
     # Sequential launches, each circuit alone:
     #  (a) one launch per piece per task with a wait after each (how the
     #      reference runtime runs GPU point tasks);
@@ -210,9 +227,14 @@ def batched_measure(C, policy_name, loops=5):
                         f"{loops} loops each, {policy_name.upper()} policy, ATOMIC reduction, "
                         "hipGraph per loop",
             "total_wires": W_total,
-            "autotuned": {"block_threads": best.block_threads, "persistent": best.persistent,
-                          "waves_per_simd": best.waves_per_simd},
+            "autotuned": best.describe(),
             "wire_updates_per_s": rep["wire_updates_per_s"], "elapsed_s": rep["elapsed_s"],
+            "default_schedule": {"config": default.describe(),
+                                 "wire_updates_per_s": rep_default["wire_updates_per_s"],
+                                 "elapsed_s": rep_default["elapsed_s"],
+                                 "probe_ms_2000_steps": default_probe_ms},
+            "autotuned_probe_ms_2000_steps": best_probe_ms,
+            "autotune_gain": rep_default["elapsed_s"] / rep["elapsed_s"],
             "sequential_per_piece": {"elapsed_s": seq_piece, "wire_updates_per_s": wu / seq_piece},
             "sequential_plain": {"elapsed_s": seq_plain, "wire_updates_per_s": wu / seq_plain},
             "sequential_fused_graph": {"elapsed_s": seq_task, "wire_updates_per_s": wu / seq_task},
@@ -444,7 +466,8 @@ def run_dsc_sweep(C, args):
     rows = []
     modes = (("atomic_global", C.REDUCE_ATOMIC, 1), ("atomic_lds", C.REDUCE_ATOMIC, 2),
              ("csr", C.REDUCE_CSR, 0), ("kahan64", C.REDUCE_CSR_KAHAN64, 0))
-    for P in (64, 256, 1024):
+    modes = tuple(m for m in modes if args.dsc_modes is None or m[0] in args.dsc_modes.split(","))
+    for P in (int(x) for x in args.dsc_pieces.split(",")):
         npp, wpp = 16384, 65536
         N, W = P * npp, P * wpp
         ctx = C.Context(0)
@@ -456,8 +479,7 @@ def run_dsc_sweep(C, args):
             ctx.distribute_charge(pieces, mode)
             ctx.synchronize()
             ctx.reset_timing()
-            reps = 20
-            for _ in range(reps):
+            for _ in range(args.dsc_reps):
                 ctx.distribute_charge(pieces, mode)
             ms, n = ctx.kernel_time(C.DISTRIBUTE_CHARGE)
             avg = ms / n

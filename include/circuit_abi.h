@@ -334,9 +334,16 @@ enum {
   CKT_OPT_WAVES_PER_SIMD = 3,    /* persistent grid: waves resident per SIMD (1..8);
                                     -1 (default): 8 for EXACT, 4 otherwise */
   CKT_OPT_DSC_ATOMIC_IMPL = 4,   /* ATOMIC reduction: 0 auto, 1 global atomics, 2 LDS */
-  CKT_OPT_FUSE_DSC = 5           /* 1 (default): in ckt_run_loops (sync 0, per_piece 0),
+  CKT_OPT_FUSE_DSC = 5,          /* 1 (default): in ckt_run_loops (sync 0, per_piece 0),
                                     batches and servers, the ATOMIC distribute_charge is
                                     done by calc_new_currents' epilogue from registers */
+  /* calc_new_currents' tile schedule (results do not depend on it): */
+  CKT_OPT_WIRES_PER_LANE = 6,    /* a tile holds workgroup size x this (1 default, 2, 4)
+                                    wires; each lane relaxes that many one after another */
+  CKT_OPT_TILE_SPAN = 7,         /* 0 (default): tiles end at piece ends; 1: adjacent pieces
+                                    of a circuit form one run, only its last tile is partial */
+  CKT_OPT_TILE_ORDER = 8         /* 0 (default): runs in the order given; 1: longest run
+                                    first; 2: interleaved, tile k of every run before tile k+1 */
 };
 int ckt_ctx_set_option(ckt_ctx_t* ctx, int option, int64_t value);
 
@@ -357,6 +364,9 @@ typedef struct ckt_batch_config {
   int32_t block_threads;    /* 64..1024, multiple of 64; 0 = autotuned/default */
   int32_t persistent;       /* 1: calc_new_currents as a persistent tile-queue kernel */
   int32_t waves_per_simd;   /* persistent grid size = CUs * 4 * waves_per_simd */
+  int32_t wires_per_lane;   /* tile schedule (CKT_OPT_WIRES_PER_LANE, _TILE_SPAN,      */
+  int32_t tile_span;        /* _TILE_ORDER); wires_per_lane 0 leaves the three as they */
+  int32_t tile_order;       /* are                                                     */
 } ckt_batch_config_t;
 
 typedef struct ckt_batch_report {
@@ -401,10 +411,18 @@ int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* ac
 int ckt_batch_loops_left(ckt_batch_t* b, uint32_t id, int32_t* loops_left);
 int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires);
 /* Time candidate schedules for the current batch and write the fastest into
- * *best.  Only calc_new_currents is timed, with `probe_steps` inner steps per
- * candidate; circuit state is restored afterwards. */
+ * *best: first the launch shape (workgroup size x plain grid or persistent
+ * queue x resident waves), then, with that shape, the tile schedule (wires
+ * per lane x tile span x tile order).  Only calc_new_currents is timed, with
+ * `probe_steps` inner steps per candidate; circuit state is restored
+ * afterwards. */
 int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
                        ckt_batch_config_t* best, double* best_ms);
+/* The probe ckt_batch_autotune times, for one configuration: milliseconds of
+ * calc_new_currents over every circuit at `probe_steps` inner steps (the
+ * faster of two launches); circuit state is restored afterwards. */
+int ckt_batch_probe(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t probe_steps,
+                    double* ms);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of circuit simulations.                      */

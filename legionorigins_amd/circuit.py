@@ -37,7 +37,7 @@ POLICY_EXACT, POLICY_RECIP, POLICY_FAST, POLICY_SCALED = 0, 1, 2, 3
 REDUCE_ATOMIC, REDUCE_CSR, REDUCE_CSR_KAHAN64 = 0, 1, 2
 PROFILE_REFERENCE, PROFILE_STABLE = 0, 1
 (OPT_STEPS, OPT_CNC_BLOCK, OPT_PERSISTENT, OPT_WAVES_PER_SIMD, OPT_DSC_ATOMIC_IMPL,
- OPT_FUSE_DSC) = range(6)
+ OPT_FUSE_DSC, OPT_WIRES_PER_LANE, OPT_TILE_SPAN, OPT_TILE_ORDER) = range(9)
 
 # CircuitNode / CircuitWire / CircuitPiece records (circuit.h:35-70).
 NODE = np.dtype([("charge", "<f4"), ("voltage", "<f4"), ("capacitance", "<f4"),
@@ -81,7 +81,13 @@ class _LoopReport(ctypes.Structure):
 class _BatchConfig(ctypes.Structure):
     _fields_ = [("policy", ctypes.c_int32), ("reduce_mode", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-                ("persistent", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32)]
+                ("persistent", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
+                ("wires_per_lane", ctypes.c_int32), ("tile_span", ctypes.c_int32),
+                ("tile_order", ctypes.c_int32)]
+
+    def describe(self):
+        return {k: getattr(self, k) for k in ("block_threads", "persistent", "waves_per_simd",
+                                              "wires_per_lane", "tile_span", "tile_order")}
 
 
 class _BatchReport(ctypes.Structure):
@@ -199,6 +205,7 @@ def _declare(L):
         "ckt_batch_loops_left": ([vp, u32, P(ctypes.c_int32)], i32),
         "ckt_batch_store": ([vp, u32, vp, vp], i32),
         "ckt_batch_autotune": ([vp, i32, i32, P(_BatchConfig), P(ctypes.c_double)], i32),
+        "ckt_batch_probe": ([vp, P(_BatchConfig), i32, P(ctypes.c_double)], i32),
         "ckt_server_create": ([i32, P(_ServerConfig), P(vp)], i32),
         "ckt_server_destroy": ([vp], i32),
         "ckt_server_run_trace": ([vp, P(_Request), u32, P(_RequestResult), P(_ServerReport)], i32),
@@ -579,9 +586,9 @@ class Batch:
 
     @staticmethod
     def config(policy=POLICY_FAST, mode=REDUCE_ATOMIC, use_graph=True, block_threads=0,
-               persistent=0, waves_per_simd=0):
+               persistent=0, waves_per_simd=0, wires_per_lane=0, tile_span=0, tile_order=0):
         return _BatchConfig(policy, mode, int(use_graph), block_threads, persistent,
-                            waves_per_simd)
+                            waves_per_simd, wires_per_lane, tile_span, tile_order)
 
     def run(self, cfg=None):
         cfg = cfg or self.config()
@@ -614,6 +621,11 @@ class Batch:
         _check(lib().ckt_batch_autotune(self.h, policy, probe_steps, ctypes.byref(best),
                                         ctypes.byref(ms)))
         return best, ms.value
+
+    def probe(self, cfg, probe_steps=500):
+        ms = ctypes.c_double()
+        _check(lib().ckt_batch_probe(self.h, ctypes.byref(cfg), probe_steps, ctypes.byref(ms)))
+        return ms.value
 
 
 @dataclass

@@ -36,14 +36,90 @@ int check(ckt_batch_t* b) {
   return CKT_OK;
 }
 
-int apply_config(ckt_batch_t* b, const ckt_batch_config_t& cfg) {
-  Engine& E = b->eng;
+int apply_config(Engine& E, const ckt_batch_config_t& cfg) {
   int st;
   if (cfg.block_threads && (st = E.set_option(OPT_CNC_BLOCK, cfg.block_threads))) return st;
   if ((st = E.set_option(OPT_PERSISTENT, cfg.persistent))) return st;
   if (cfg.waves_per_simd && (st = E.set_option(OPT_WAVES_PER_SIMD, cfg.waves_per_simd))) return st;
+  if (cfg.wires_per_lane) {
+    if ((st = E.set_option(OPT_WIRES_PER_LANE, cfg.wires_per_lane))) return st;
+    if ((st = E.set_option(OPT_TILE_SPAN, cfg.tile_span))) return st;
+    if ((st = E.set_option(OPT_TILE_ORDER, cfg.tile_order))) return st;
+  }
   return CKT_OK;
 }
+
+int apply_config(ckt_batch_t* b, const ckt_batch_config_t& cfg) { return apply_config(b->eng, cfg); }
+
+// The probes of ckt_batch_autotune / ckt_batch_probe: calc_new_currents over
+// every circuit at a shortened step count, timed with events on the engine
+// stream.  The constructor saves each circuit's currents and internal
+// voltages (one contiguous region of its slab) and the engine options; the
+// destructor puts both back.
+class Prober {
+ public:
+  explicit Prober(Engine& E) : E_(E) {}
+  int begin(int32_t probe_steps) {
+    for (int o = 0; o < kOpts; o++) opts_[o] = E_.option(kOptIds[o]);
+    began_ = true;
+    hipStream_t s = E_.stream();
+    for (Circuit* c : E_.slots()) {
+      const size_t bytes = c->lay.ind - c->lay.cur;
+      void* p = nullptr;
+      CKT_HIP(hipMalloc(&p, bytes));
+      saved_.push_back(p);
+      CKT_HIP(hipMemcpyAsync(p, (uint8_t*)c->slab + c->lay.cur, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    E_.all_refs(refs_, false);
+    CKT_HIP(hipEventCreate(&a_));
+    CKT_HIP(hipEventCreate(&z_));
+    return E_.set_option(OPT_STEPS, probe_steps);
+  }
+  // The faster of two launches under `cfg`'s launch shape and schedule.
+  int time(int32_t policy, const ckt_batch_config_t& cfg, double* ms) {
+    int st;
+    if ((st = apply_config(E_, cfg))) return st;
+    double t = 1e30;
+    for (int rep = 0; rep < 2; rep++) {
+      if ((st = E_.sync())) return st;
+      CKT_HIP(hipEventRecord(a_, E_.stream()));
+      if ((st = E_.cnc(refs_, policy))) return st;
+      CKT_HIP(hipEventRecord(z_, E_.stream()));
+      if ((st = E_.sync())) return st;
+      float f = 0;
+      CKT_HIP(hipEventElapsedTime(&f, a_, z_));
+      t = std::min(t, (double)f);
+    }
+    *ms = t;
+    return CKT_OK;
+  }
+  ~Prober() {
+    if (a_) (void)hipEventDestroy(a_);
+    if (z_) (void)hipEventDestroy(z_);
+    if (began_)
+      for (int o = 0; o < kOpts; o++) (void)E_.set_option(kOptIds[o], opts_[o]);
+    hipStream_t s = E_.stream();
+    for (size_t i = 0; i < saved_.size(); i++) {
+      Circuit* c = E_.slots()[i];
+      (void)hipMemcpyAsync((uint8_t*)c->slab + c->lay.cur, saved_[i], c->lay.ind - c->lay.cur,
+                           hipMemcpyDeviceToDevice, s);
+    }
+    (void)hipStreamSynchronize(s);
+    for (void* p : saved_) (void)hipFree(p);
+  }
+
+ private:
+  static constexpr int kOpts = 7;
+  static constexpr int kOptIds[kOpts] = {OPT_STEPS,          OPT_CNC_BLOCK, OPT_PERSISTENT,
+                                         OPT_WAVES_PER_SIMD, OPT_WIRES_PER_LANE,
+                                         OPT_TILE_SPAN,      OPT_TILE_ORDER};
+  Engine& E_;
+  std::vector<void*> saved_;
+  std::vector<PieceRef> refs_;
+  int64_t opts_[kOpts] = {};
+  bool began_ = false;
+  hipEvent_t a_ = nullptr, z_ = nullptr;
+};
 
 }  // namespace
 
@@ -270,10 +346,10 @@ int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* 
   return b->eng.store(id, nodes, wires);
 }
 
-// Times calc_new_currents over every circuit of the batch for each candidate
-// schedule (workgroup size x plain grid or persistent queue x residency),
-// with a shortened inner step count, and keeps the fastest.  The wire state
-// changed by the probes is saved before and restored after.
+// Two stages, each keeping the fastest candidate: the launch shape
+// (workgroup size x plain grid or persistent queue x resident waves) with the
+// default tile schedule, then the tile schedule (wires per lane x span x
+// order) with that shape.
 int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
                        ckt_batch_config_t* best, double* best_ms) {
   int st = check(b);
@@ -282,72 +358,51 @@ int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
   if (probe_steps < 1) probe_steps = 500;
   Engine& E = b->eng;
   if (E.count() == 0) return fail(CKT_ERR_STATE, "empty batch");
-  hipStream_t s = E.stream();
-  // Save the mutable wire state (currents and internal voltages are one
-  // contiguous region of each slab).
-  std::vector<void*> saved;
-  for (Circuit* c : E.slots()) {
-    const size_t bytes = c->lay.ind - c->lay.cur;
-    void* p = nullptr;
-    CKT_HIP(hipMalloc(&p, bytes));
-    saved.push_back(p);
-    CKT_HIP(hipMemcpyAsync(p, (uint8_t*)c->slab + c->lay.cur, bytes, hipMemcpyDeviceToDevice, s));
-  }
-  const int64_t steps0 = E.option(OPT_STEPS), block0 = E.option(OPT_CNC_BLOCK),
-                pers0 = E.option(OPT_PERSISTENT), waves0 = E.option(OPT_WAVES_PER_SIMD);
-  std::vector<PieceRef> refs;
-  E.all_refs(refs, false);
-  E.set_option(OPT_STEPS, probe_steps);
-  hipEvent_t a, z;
-  CKT_HIP(hipEventCreate(&a));
-  CKT_HIP(hipEventCreate(&z));
+  Prober P(E);
+  if ((st = P.begin(probe_steps))) return st;
   double best_t = 1e30;
-  ckt_batch_config_t bc{policy, CKT_REDUCE_ATOMIC, 1, 256, 0, 8};
-  const int blocks[4] = {64, 128, 256, 512};
-  for (int pers = 0; pers <= 1 && !st; pers++) {
-    for (int bi = 0; bi < 4 && !st; bi++) {
+  ckt_batch_config_t bc{policy, CKT_REDUCE_ATOMIC, 1, 256, 0, 8, 1, 0, 0};
+  for (int pers = 0; pers <= 1; pers++)
+    for (int block : {64, 128, 256, 512})
       for (int waves : {4, 8}) {
         if (!pers && waves != 8) continue;
-        E.set_option(OPT_CNC_BLOCK, blocks[bi]);
-        E.set_option(OPT_PERSISTENT, pers);
-        E.set_option(OPT_WAVES_PER_SIMD, waves);
-        double t = 1e30;
-        for (int rep = 0; rep < 2 && !st; rep++) {
-          if ((st = E.sync())) break;
-          hipEventRecord(a, s);
-          st = E.cnc(refs, policy);
-          hipEventRecord(z, s);
-          if (!st) st = E.sync();
-          float ms = 0;
-          if (!st) hipEventElapsedTime(&ms, a, z);
-          t = std::min(t, (double)ms);
-        }
-        if (!st && t < best_t) {
-          best_t = t;
-          bc.block_threads = blocks[bi];
-          bc.persistent = pers;
-          bc.waves_per_simd = waves;
-        }
+        ckt_batch_config_t c = bc;
+        c.block_threads = block;
+        c.persistent = pers;
+        c.waves_per_simd = waves;
+        double t;
+        if ((st = P.time(policy, c, &t))) return st;
+        if (t < best_t) best_t = t, bc = c;
       }
-    }
-  }
-  hipEventDestroy(a);
-  hipEventDestroy(z);
-  E.set_option(OPT_STEPS, steps0);
-  E.set_option(OPT_CNC_BLOCK, block0);
-  E.set_option(OPT_PERSISTENT, pers0);
-  E.set_option(OPT_WAVES_PER_SIMD, waves0);
-  for (size_t i = 0; i < saved.size(); i++) {
-    Circuit* c = E.slots()[i];
-    hipMemcpyAsync((uint8_t*)c->slab + c->lay.cur, saved[i], c->lay.ind - c->lay.cur,
-                   hipMemcpyDeviceToDevice, s);
-  }
-  hipStreamSynchronize(s);
-  for (void* p : saved) hipFree(p);
-  if (st) return st;
+  const ckt_batch_config_t shape = bc;
+  for (int lane : {1, 2, 4})
+    for (int span : {0, 1})
+      for (int order : {0, 1, 2}) {
+        if (lane == 1 && span == 0 && order == 0) continue;  // timed in the first stage
+        ckt_batch_config_t c = shape;
+        c.wires_per_lane = lane;
+        c.tile_span = span;
+        c.tile_order = order;
+        double t;
+        if ((st = P.time(policy, c, &t))) return st;
+        if (t < best_t) best_t = t, bc = c;
+      }
   *best = bc;
   if (best_ms) *best_ms = best_t;
   return CKT_OK;
+}
+
+int ckt_batch_probe(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t probe_steps,
+                    double* ms) {
+  int st = check(b);
+  if (st) return st;
+  if (!config || !ms) return fail(CKT_ERR_INVALID_ARG, "null argument");
+  if (probe_steps < 1) return fail(CKT_ERR_INVALID_ARG, "probe_steps must be positive");
+  Engine& E = b->eng;
+  if (E.count() == 0) return fail(CKT_ERR_STATE, "empty batch");
+  Prober P(E);
+  if ((st = P.begin(probe_steps))) return st;
+  return P.time(config->policy, *config, ms);
 }
 
 }  // extern "C"

@@ -1,5 +1,6 @@
 // ckt_engine.cpp -- device-resident circuits, tile lists and task launches.
 #include "ckt_engine.h"
+#include "ckt_index.h"
 
 #include <algorithm>
 #include <cstring>
@@ -598,7 +599,7 @@ int Engine::upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n,
   return CKT_OK;
 }
 
-enum TileKind { TK_WIRES = 1, TK_NODES = 2, TK_CSR = 3, TK_PIECES = 4 };
+enum TileKind { TK_WIRES = 1, TK_NODES = 2, TK_CSR = 3, TK_PIECES = 4, TK_CNC = 5 };
 // update_voltages: 256 lanes x 8 nodes per tile.
 constexpr int kUpvTile = 2048;
 // Node bytes update_voltages moves per node (4 arrays read, 2 written) and the
@@ -610,6 +611,10 @@ constexpr uint64_t kLastLevelCacheBytes = 256ull << 20;
 int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out) {
   uint64_t key = fnv(1469598103934665603ull, &kind, sizeof(kind));
   key = fnv(key, &block, sizeof(block));
+  if (kind == TK_CNC) {
+    const int sched[3] = {wires_per_lane_, tile_span_, tile_order_};
+    key = fnv(key, sched, sizeof(sched));
+  }
   key = fnv(key, &generation_, sizeof(generation_));
   key = fnv(key, refs.data(), refs.size() * sizeof(PieceRef));
   auto it = tile_cache_.find(key);
@@ -639,6 +644,14 @@ int Engine::tiles_for(int kind, const std::vector<PieceRef>& refs, int block, De
       for (uint32_t off = 0; off < N; off += (uint32_t)block)
         t.push_back(Tile{s, off, std::min<uint32_t>((uint32_t)block, N - off), 0});
     }
+  } else if (kind == TK_CNC) {
+    std::vector<Tile> pieces;
+    pieces.reserve(refs.size());
+    for (const PieceRef& r : refs) {
+      const ckt_piece_t& p = slots_[r.slot]->pieces[r.piece];
+      pieces.push_back(Tile{r.slot, p.first_wire, p.num_wires, r.piece});
+    }
+    t = cnc_schedule<Tile>(pieces, (uint32_t)block * (uint32_t)wires_per_lane_, tile_span_, tile_order_);
   } else {
     for (const PieceRef& r : refs) {
       const ckt_piece_t& p = slots_[r.slot]->pieces[r.piece];
@@ -720,7 +733,7 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
   int st;
   if ((st = ensure_table())) return st;
   DevBuf t;
-  if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
+  if ((st = tiles_for(TK_CNC, refs, cnc_block_, &t))) return st;
   hipEvent_t a;
   if ((st = timed_begin(CKT_CALC_NEW_CURRENTS, &a))) return st;
   hipError_t e;
@@ -755,7 +768,7 @@ int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
   if (tile_cache_.size() + 8 > 64) drop_caches();
   if ((st = ensure_table())) return st;
   DevBuf t;
-  if ((st = tiles_for(TK_WIRES, refs, cnc_block_, &t))) return st;
+  if ((st = tiles_for(TK_CNC, refs, cnc_block_, &t))) return st;
   if (mode == CKT_REDUCE_ATOMIC && !fuse_dsc_) {
     if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
     LdsPlan plan;
@@ -983,8 +996,9 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
   int st;
   if ((st = ensure_table())) return st;
   uint64_t key = fnv(1469598103934665603ull, refs.data(), refs.size() * sizeof(PieceRef));
-  const int64_t cfg[9] = {policy, mode, steps_, cnc_block_, persistent_, waves_per_simd_,
-                          dsc_atomic_impl_, fuse_dsc_, (int64_t)generation_};
+  const int64_t cfg[12] = {policy,           mode,       steps_,          cnc_block_,
+                           persistent_,      waves_per_simd_, dsc_atomic_impl_, fuse_dsc_,
+                           wires_per_lane_,  tile_span_, tile_order_,     (int64_t)generation_};
   key = fnv(key, cfg, sizeof(cfg));
   auto it = graphs_.find(key);
   hipGraphExec_t exec = nullptr;
@@ -1036,6 +1050,18 @@ int Engine::set_option(int opt, int64_t v) {
       dsc_atomic_impl_ = (int)v;
       return CKT_OK;
     case OPT_FUSE_DSC: fuse_dsc_ = v ? 1 : 0; return CKT_OK;
+    case OPT_WIRES_PER_LANE:
+      if (v != 1 && v != 2 && v != 4) return fail(CKT_ERR_INVALID_ARG, "wires per lane must be 1, 2 or 4");
+      wires_per_lane_ = (int)v;
+      return CKT_OK;
+    case OPT_TILE_SPAN:
+      if (v < 0 || v > 1) return fail(CKT_ERR_INVALID_ARG, "tile span must be 0 or 1");
+      tile_span_ = (int)v;
+      return CKT_OK;
+    case OPT_TILE_ORDER:
+      if (v < 0 || v > 2) return fail(CKT_ERR_INVALID_ARG, "tile order must be 0, 1 or 2");
+      tile_order_ = (int)v;
+      return CKT_OK;
     default: return fail(CKT_ERR_INVALID_ARG, "unknown option");
   }
 }
@@ -1048,6 +1074,9 @@ int64_t Engine::option(int opt) const {
     case OPT_WAVES_PER_SIMD: return waves_per_simd_;
     case OPT_DSC_ATOMIC_IMPL: return dsc_atomic_impl_;
     case OPT_FUSE_DSC: return fuse_dsc_;
+    case OPT_WIRES_PER_LANE: return wires_per_lane_;
+    case OPT_TILE_SPAN: return tile_span_;
+    case OPT_TILE_ORDER: return tile_order_;
     default: return -1;
   }
 }

@@ -93,6 +93,9 @@ enum EngineOption {
   OPT_WAVES_PER_SIMD = 3,
   OPT_DSC_ATOMIC_IMPL = 4,  // 0 auto, 1 global atomics, 2 LDS per piece
   OPT_FUSE_DSC = 5,         // loop drivers: ATOMIC distribute_charge fused into calc_new_currents
+  OPT_WIRES_PER_LANE = 6,   // calc_new_currents tile = workgroup size x this (1, 2, 4) wires
+  OPT_TILE_SPAN = 7,        // 0: tiles within a piece; 1: within runs of adjacent pieces
+  OPT_TILE_ORDER = 8,       // 0: as given; 1: longest runs first; 2: runs interleaved
 };
 
 class Engine {
@@ -218,6 +221,9 @@ class Engine {
   int waves_per_simd_ = -1;  // -1: chosen per policy
   int dsc_atomic_impl_ = 0;
   int fuse_dsc_ = 1;
+  int wires_per_lane_ = 1;
+  int tile_span_ = 0;
+  int tile_order_ = 0;
   bool timing_ = false;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_;

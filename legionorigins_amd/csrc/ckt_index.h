@@ -8,6 +8,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <vector>
+
 namespace ckt {
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
@@ -53,6 +56,48 @@ __host__ __device__ inline void lds_subrange(uint32_t num_wires, uint32_t splits
 // [4 (g 256 + lane), +4) for g = 0, 1 (two lane-interleaved groups of 4).
 __host__ __device__ inline uint32_t upv_group_first(uint32_t g, uint32_t lane) {
   return 4u * (g * 256u + lane);
+}
+
+// calc_new_currents' tile list (host side).  `pieces` holds one element per
+// piece, {circuit, first wire, wires, piece}.  A run is a range of
+// consecutive wires of one circuit: a piece, or with `span` adjacent pieces
+// of one circuit merged, so that only a run's last tile is partial.  Tiles
+// hold at most `tile` wires and are listed run by run in the order given
+// (order 0), longest run first (1, stable), or interleaved (2: tile k of
+// every run before tile k + 1 of any).  T is any struct with those four
+// uint32_t members.
+template <class T>
+std::vector<T> cnc_schedule(const std::vector<T>& pieces, uint32_t tile, int span, int order) {
+  std::vector<T> runs;
+  for (const T& p : pieces) {
+    if (!p.count) continue;
+    if (span && !runs.empty() && runs.back().circuit == p.circuit &&
+        (uint64_t)runs.back().first + runs.back().count == p.first)
+      runs.back().count += p.count;
+    else
+      runs.push_back(p);
+  }
+  if (order == 1)
+    std::stable_sort(runs.begin(), runs.end(), [](const T& a, const T& b) { return a.count > b.count; });
+  std::vector<T> t;
+  auto tile_of = [&](const T& r, uint32_t off) {
+    T x = r;
+    x.first = r.first + off;
+    x.count = std::min(tile, r.count - off);
+    return x;
+  };
+  if (order == 2) {
+    for (uint64_t off = 0;; off += tile) {
+      const size_t before = t.size();
+      for (const T& r : runs)
+        if (off < r.count) t.push_back(tile_of(r, (uint32_t)off));
+      if (t.size() == before) break;
+    }
+  } else {
+    for (const T& r : runs)
+      for (uint64_t off = 0; off < r.count; off += tile) t.push_back(tile_of(r, (uint32_t)off));
+  }
+  return t;
 }
 
 }  // namespace ckt

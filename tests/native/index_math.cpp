@@ -95,7 +95,103 @@ static void check_upv_groups() {
   }
 }
 
+// calc_new_currents' tile schedule, over random sets of pieces (circuits of
+// 1-12 contiguous pieces of 0-3000 wires, some pieces left out, some lists
+// out of wire order):
+//  - every wire of every listed piece is in exactly one tile, and each tile
+//    holds 1..tile wires of one circuit;
+//  - without span, tiles stay inside one piece and number the sum of
+//    ceil(piece / tile); longest-first lists the tiles' pieces by
+//    non-increasing size; interleaved lists them by non-decreasing tile
+//    index within their piece;
+//  - with span, a circuit whose pieces are all listed in wire order gets
+//    ceil(wires / tile) tiles, so at most one partial.
+struct Run {
+  uint32_t circuit, first, count, piece;
+};
+static void check_cnc_schedule() {
+  std::mt19937 rng(7);
+  for (int trial = 0; trial < 3000; trial++) {
+    std::vector<Run> pieces;
+    std::vector<uint32_t> wires;   // per circuit
+    std::vector<bool> complete;    // every piece listed, in wire order
+    const uint32_t circuits = 1 + rng() % 5;
+    for (uint32_t c = 0; c < circuits; c++) {
+      const uint32_t np = 1 + rng() % 12;
+      uint32_t first = 0;
+      bool all = true;
+      std::vector<Run> mine;
+      for (uint32_t p = 0; p < np; p++) {
+        const uint32_t n = rng() % 7 == 0 ? 0 : rng() % 3000;
+        if (rng() % 6 != 0) mine.push_back(Run{c, first, n, p});
+        else all = false;
+        first += n;
+      }
+      if (rng() % 4 == 0 && mine.size() > 1 && mine[0].count && mine.back().count) {
+        std::swap(mine[0], mine.back());
+        all = false;
+      }
+      pieces.insert(pieces.end(), mine.begin(), mine.end());
+      wires.push_back(first);
+      complete.push_back(all);
+    }
+    // the listed piece holding wire w of circuit c
+    auto piece_of = [&](uint32_t c, uint32_t w) -> const Run* {
+      for (const Run& p : pieces)
+        if (p.circuit == c && w >= p.first && w - p.first < p.count) return &p;
+      return nullptr;
+    };
+    const uint32_t sizes[4] = {64, 256, 512, 2048};
+    const uint32_t tile = sizes[rng() % 4];
+    for (int span = 0; span <= 1; span++)
+      for (int order = 0; order <= 2; order++) {
+        const std::vector<Run> t = cnc_schedule<Run>(pieces, tile, span, order);
+        std::vector<std::vector<int>> seen(circuits);
+        for (uint32_t c = 0; c < circuits; c++) seen[c].assign(wires[c], 0);
+        std::vector<size_t> per_circuit(circuits, 0);
+        bool ok = true;
+        for (const Run& x : t) {
+          const bool in = x.circuit < circuits && x.count >= 1 && x.count <= tile &&
+                          (uint64_t)x.first + x.count <= wires[x.circuit];
+          CHECK(in, "trial %d: tile {%u, %u, %u} malformed", trial, x.circuit, x.first, x.count);
+          if (!in) { ok = false; continue; }
+          per_circuit[x.circuit]++;
+          for (uint32_t k = 0; k < x.count; k++) seen[x.circuit][x.first + k]++;
+        }
+        if (!ok) continue;
+        for (const Run& p : pieces)
+          for (uint32_t k = 0; k < p.count; k++)
+            CHECK(seen[p.circuit][p.first + k] == 1, "trial %d span %d order %d: wire %u covered %d times",
+                  trial, span, order, p.first + k, seen[p.circuit][p.first + k]);
+        if (!span) {
+          size_t want = 0;
+          for (const Run& p : pieces) want += (p.count + tile - 1) / tile;
+          CHECK(t.size() == want, "trial %d order %d: %zu tiles, expected %zu", trial, order, t.size(), want);
+          uint32_t prev_size = UINT32_MAX, prev_k = 0;
+          for (const Run& x : t) {
+            const Run* p = piece_of(x.circuit, x.first);
+            CHECK(p && x.first + x.count <= p->first + p->count, "trial %d: tile crosses a piece end", trial);
+            if (!p) continue;
+            const uint32_t k = (x.first - p->first) / tile;
+            CHECK((x.first - p->first) % tile == 0, "trial %d: tile not at a multiple of the tile size", trial);
+            if (order == 1) CHECK(p->count <= prev_size, "trial %d: piece sizes not non-increasing", trial);
+            if (order == 2) CHECK(k >= prev_k, "trial %d: interleaved tile index decreases", trial);
+            prev_size = p->count;
+            prev_k = k;
+          }
+        } else {
+          for (uint32_t c = 0; c < circuits; c++)
+            if (complete[c])
+              CHECK(per_circuit[c] == (wires[c] + tile - 1) / tile,
+                    "trial %d order %d: circuit %u of %u wires in %zu tiles", trial, order, c, wires[c],
+                    per_circuit[c]);
+        }
+      }
+  }
+}
+
 int main() {
+  check_cnc_schedule();
   check_xcd_tile();
   check_piece_of_block();
   check_lds_subrange();

@@ -94,6 +94,9 @@ def test_autotune_restores_state(C):
            for (P, npp, wpp, pct, seed, loops) in SHAPES]
     best, ms = b.autotune(C.POLICY_EXACT, probe_steps=50)
     assert best.block_threads in (64, 128, 256, 512) and ms > 0
+    assert best.wires_per_lane in (1, 2, 4) and best.tile_span in (0, 1)
+    assert best.tile_order in (0, 1, 2)
+    assert b.probe(best, probe_steps=50) > 0
     best.policy, best.reduce_mode = C.POLICY_EXACT, C.REDUCE_CSR
     b.run(best)
     for cid, shape in zip(ids, SHAPES):
@@ -102,6 +105,31 @@ def test_autotune_restores_state(C):
         n1, w1 = alone(C, shape, C.POLICY_EXACT, C.REDUCE_CSR)
         assert_bit_identical(nodes["voltage"], n1["voltage"], "node voltage")
         assert_bit_identical(wires["current"], w1["current"], "wire current")
+    b.close()
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("sched", [(2, 0, 0), (4, 1, 1), (1, 1, 2), (2, 1, 2)],
+                         ids=["2-per-lane", "4-per-lane-span-longest", "span-interleaved",
+                              "2-per-lane-span-interleaved"])
+def test_tile_schedules_bit_identical(C, O, sched, persistent):
+    # calc_new_currents' tile schedule (wires per lane, tiles spanning
+    # adjacent pieces, tile order) moves work between workgroups only: with
+    # EXACT + CSR every circuit stays bit-identical to the oracle's loop.
+    lane, span, order = sched
+    b = C.Batch(0)
+    ids = [b.add_generated(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE), loops)
+           for (P, npp, wpp, pct, seed, loops) in SHAPES]
+    cfg = C.Batch.config(policy=C.POLICY_EXACT, mode=C.REDUCE_CSR, block_threads=128,
+                         persistent=persistent, waves_per_simd=4, wires_per_lane=lane,
+                         tile_span=span, tile_order=order)
+    b.run(cfg)
+    for cid, (P, npp, wpp, pct, seed, loops) in zip(ids, SHAPES):
+        nodes, wires = b.store(cid, P * npp, P * wpp)
+        ref = O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE)
+        O.run(ref, loops)
+        assert nodes.tobytes() == ref.nodes.tobytes()
+        assert wires.tobytes() == ref.wires.tobytes()
     b.close()
 
 

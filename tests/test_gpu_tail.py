@@ -116,3 +116,29 @@ def test_growth_regimes_overflowing_near_the_end(C, O, gain, steps, start):
         f"{_mixed_wires(want)} mixed")
     assert 0 < nonfinite < len(want) or start == (-150, -96)
     _check_all_policies(C, circ, want, steps, f"gain {gain}")
+
+
+@pytest.mark.parametrize("steps", [17, 40, 10000])
+def test_wires_that_skip_the_tail(C, O, steps):
+    # Contracting wires with parameters and starting values at the edges of
+    # the region where the policies skip the reference-order tail (R at
+    # 2^-20..2^20, values up to 2^40, step norms up to 1/2), mixed with
+    # wires just outside it (values of 2^41, R of 2^21), which take the tail:
+    # every policy against the oracle, as above.
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from test_tail_guard import draw
+    c = O.generate(1, 256, 4096, 100, 3, O.PROFILE_STABLE)
+    w, n = c.wires, c.nodes
+    W = len(w)
+    rng = np.random.default_rng(steps)
+    I, V, vin, vout, L, R, Cap = draw(rng, W)
+    w["current"], w["voltage"] = I, V
+    w["inductance"], w["resistance"], w["capacitance"] = L, R, Cap
+    n["voltage"] = (rng.choice([-1, 1], len(n)) * 2.0 ** rng.uniform(0, 40, len(n))).astype(np.float32)
+    out = rng.random(W) < 0.2
+    w["current"][out, 0] = np.float32(2.0**41)
+    w["resistance"][rng.random(W) < 0.1] = np.float32(2.0**21)
+    want = _relax_oracle(O, c, steps)
+    nonfinite = int((~np.isfinite(want["current"])).any(1).sum())
+    say(f"[tail] guard edges, {steps} steps: {nonfinite} of {W} wires non-finite")
+    _check_all_policies(C, c, want, steps, f"guard edges {steps}")

@@ -46,6 +46,55 @@ def test_one_step_from_rest_is_ohms_law(O):
     assert not V[:, 1:8].any()
 
 
+def _steps_in_order(w, vin, vout, steps, inductive, capacitive):
+    """The reference's step (circuit.cc:387-401) in numpy float32, one
+    rounding per operation, with the inductive and capacitive terms formed
+    by the given functions."""
+    I0, V0 = w["current"].copy(), w["voltage"].copy()
+    L, R, C = w["inductance"][:, None], w["resistance"][:, None], w["capacitance"][:, None]
+    I = I0.copy()
+    v = np.concatenate([vin[:, None], V0, vout[:, None]], axis=1)
+    for _ in range(steps):
+        I = ((v[:, 1:] - v[:, :-1]) - inductive(L, I, I0)) / R
+        v = v.copy()
+        v[:, 1:-1] = V0 + capacitive(C, I[:, :-1] - I[:, 1:])
+    return I, v[:, 1:-1]
+
+
+def test_inductive_and_capacitive_terms_in_reference_order(O):
+    # From a state where the currents differ from the loop's starting
+    # currents (so the inductive term is nonzero, unlike the one-step test
+    # above), three steps of the oracle equal the reference's expressions
+    # evaluated in their written order, L*(I - I0)/dt and dt*(I_i - I_i+1)/C,
+    # bit for bit; each plausible reordering of either term gives different
+    # bits on this state, so the comparison tells them apart.
+    c = _stable(O, P=1, npp=256, wpp=4096, pct=100, seed=5)
+    w, n = c.wires, c.nodes
+    rng = np.random.default_rng(5)
+    w["current"] = rng.uniform(-1, 1, w["current"].shape).astype(np.float32)
+    w["voltage"] = rng.uniform(-1, 1, w["voltage"].shape).astype(np.float32)
+    w["inductance"] = rng.uniform(1e-7, 1e-6, len(w)).astype(np.float32)
+    before = w.copy()
+    vin = n["voltage"][w["in_ptr"]].copy()
+    vout = n["voltage"][w["out_ptr"]].copy()
+    O.calc_new_currents(c, steps=3)
+    ind = lambda L, I, I0: (L * (I - I0)) / DT32
+    cap = lambda C, dI: (DT32 * dI) / C
+    I, V = _steps_in_order(before, vin, vout, 3, ind, cap)
+    assert np.array_equal(c.wires["current"], I)
+    assert np.array_equal(c.wires["voltage"], V)
+    others = [
+        (lambda L, I, I0: L * ((I - I0) / DT32), cap),
+        (lambda L, I, I0: (L / DT32) * (I - I0), cap),
+        (lambda L, I, I0: (L * (I - I0)) * (f32(1) / DT32), cap),
+        (ind, lambda C, dI: DT32 * (dI / C)),
+        (ind, lambda C, dI: (DT32 / C) * dI),
+    ]
+    for k, (a, b) in enumerate(others):
+        I2, V2 = _steps_in_order(before, vin, vout, 3, a, b)
+        assert not (np.array_equal(I2, I) and np.array_equal(V2, V)), f"reordering {k} not told apart"
+
+
 def test_cpu_literal_seeding_ignores_in_node(O):
     a = _stable(O)
     b = _stable(O)
