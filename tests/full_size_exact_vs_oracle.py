@@ -6,7 +6,10 @@ wire compared: EXACT + CSR bit for bit; SCALED and FAST with the fused ATOMIC
 reduction (the benchmark's configurations) by non-finite class and within 1e-5
 (node voltages elementwise, wire currents and internal voltages per wire,
 relative to the wire's largest segment value).  Both profiles.  The oracle
-needs about 1.7 core-hours per profile (~7 minutes on 16 threads).
+needs about 1.7 core-hours per profile (~7 minutes on 16 threads), so this
+script is run by hand and pytest does not collect it; the collected 10-loop
+check at this size is test_gpu_fullsize.py's loop-by-loop comparison, which
+runs the oracle's task bodies on a wire sample and every node each loop.
 
     python tests/full_size_exact_vs_oracle.py [--profile reference|stable] [--loops N]
 """
