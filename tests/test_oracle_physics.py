@@ -165,6 +165,40 @@ def test_fixed_point_is_backward_euler_rlc_step(O):
     assert worst32 < 1e-5, worst32
 
 
+@pytest.mark.parametrize("profile", [0, 1], ids=["reference", "stable"])
+def test_fp64_steps_match_exact_rational_arithmetic(O, profile):
+    # The FP64 twin of calc_new_currents over k <= 3 steps against the same
+    # recurrence (circuit.cc:387-401, GPU seeding circuit_gpu.cu:107-112) in
+    # exact rational arithmetic from the same inputs, for 40 wires with
+    # nonzero starting currents and voltages: per wire, within 1e-12 of the
+    # wire's largest value.
+    from fractions import Fraction as Fr
+    c = O.generate(1, 64, 256, 100, 9, profile)
+    rng = np.random.default_rng(9)
+    c.wires["current"] = rng.uniform(-1, 1, c.wires["current"].shape).astype(np.float32)
+    c.wires["voltage"] = rng.uniform(-1, 1, c.wires["voltage"].shape).astype(np.float32)
+    for k in (1, 2, 3):
+        n64, w64 = O.promote(c)
+        O.calc_new_currents_f64(n64, w64, 0, 40, steps=k)
+        dt = Fr(float(np.float32(1e-6)))
+        for wi in range(40):
+            w = c.wires[wi]
+            L, R, C = Fr(float(w["inductance"])), Fr(float(w["resistance"])), Fr(float(w["capacitance"]))
+            I0 = [Fr(float(x)) for x in w["current"]]
+            V0 = [Fr(float(x)) for x in w["voltage"]]
+            v = ([Fr(float(c.nodes["voltage"][w["in_ptr"]]))] + V0 +
+                 [Fr(float(c.nodes["voltage"][w["out_ptr"]]))])
+            I = list(I0)
+            for _ in range(k):
+                I = [((v[i + 1] - v[i]) - L * (I[i] - I0[i]) / dt) / R for i in range(10)]
+                v = [v[0]] + [V0[i] + dt * (I[i] - I[i + 1]) / C for i in range(9)] + [v[10]]
+            got_i, got_v = w64[wi]["current"], w64[wi]["voltage"]
+            scale = max(max(abs(float(x)) for x in I), max(abs(float(x)) for x in v[1:10]))
+            err = max(max(abs(float(I[i]) - got_i[i]) for i in range(10)),
+                      max(abs(float(v[i + 1]) - got_v[i]) for i in range(9)))
+            assert err <= 1e-12 * scale, (profile, k, wi, err, scale)
+
+
 def test_fp64_limit_equals_stepwise_fp64(O):
     # The backward-Euler solve is the limit of the 10000-step relaxation in
     # the stable profile: the step-by-step FP64 oracle reaches it.
