@@ -2,7 +2,7 @@
 (64 pieces x 16384 nodes x 65536 wires, 95% in-piece, seed 12345) for its 10
 loops on the GPU against the oracle's FP32 restatement of the reference tasks
 (circuit.cc:352-460, sequential reduction order) on the host, every node and
-wire compared: EXACT + CSR bit for bit; SCALED and FAST with the fused ATOMIC
+wire compared: EXACT + CSR bit for bit; SCALED, FAST and RECIP with the fused ATOMIC
 reduction (the benchmark's configurations) by non-finite class and within 1e-5
 (node voltages elementwise, wire currents and internal voltages per wire,
 relative to the wire's largest segment value).  Both profiles.  The oracle
@@ -44,7 +44,8 @@ def main():
         nodes, wires = ctx.store()
     print(f"[full size] GPU EXACT+CSR {a.loops} loops done in {time.time() - t0:.1f} s", flush=True)
     fast = {}
-    for name, policy in (("scaled+atomic", C.POLICY_SCALED), ("fast+atomic", C.POLICY_FAST)):
+    for name, policy in (("scaled+atomic", C.POLICY_SCALED), ("fast+atomic", C.POLICY_FAST),
+                         ("recip+atomic", C.POLICY_RECIP)):
         with C.Context(0) as ctx:
             ctx.generate(C.GenParams(P, npp, wpp, pct, seed, prof))
             ctx.run_loops(a.loops, policy=policy, mode=C.REDUCE_ATOMIC)
