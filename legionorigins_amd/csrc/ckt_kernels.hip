@@ -183,20 +183,23 @@ __device__ __forceinline__ bool whole_loop_in_reference_order(const float (&I)[k
 // bounded as below skips the tail: neither the reference's iteration nor the
 // policy's can leave the finite range, so both end finite, and the policy's
 // own steps run to the end.  With the step's infinity-norm at most 1/2 (the
-// substituted update x <- K + A x of FAST below), every current stays within
-// max(|x_1|, 2|K|) <= 2^62 when |I0|, |V0|, |vin|, |vout| <= 2^40 and
-// 2^-20 <= R <= 2^20 (then g <= 1, d <= 1/4, b = dR <= 2^18, 1/C <= 2^38,
-// L <= R dt); every intermediate of the reference's step then stays below
-// 2^104: voltages below 2^81, (v_{i+1} - v_i) - L (I - I0) / dt below 2^84,
-// its quotient by R below 2^104, dt (I_i - I_{i+1}) / C below 2^81.  The
-// stable profile's wires all pass; the reference profile's (g ~ 1e4..1e5)
-// none.  A NaN anywhere fails a comparison and keeps the tail.
+// substituted update x <- K + A x of FAST below; |d| in the norm, so a
+// negative capacitance is bounded too), every current stays within
+// max(|x_1|, 2|K|) <= 2^63 when |I0|, |V0|, |vin|, |vout| <= 2^40 and
+// 2^-20 <= R <= 2^20 (then |g| <= 1, |d| <= 1/4, |b| = |d| R <= 2^18,
+// |1/C| <= 2^38, |L| <= R dt); every intermediate of the reference's step
+// then stays below 2^104: voltages below 2^82, (v_{i+1} - v_i) - L (I - I0) /
+// dt below 2^84, its quotient by R below 2^104, dt (I_i - I_{i+1}) / C below
+// 2^82.  The stable profile's wires all pass; the reference profile's
+// (g ~ 1e4..1e5) none.  A NaN anywhere fails a comparison and keeps the
+// tail.
 __device__ __forceinline__ bool tail_stays_finite(const float (&I)[kSeg], const float (&V)[kSeg - 1],
                                                   float vin, float vout, float L, float R,
                                                   float C) {
   const float dt = CKT_DT_F;
   const float g = (L / dt) / R, d = (1.0f / R) * (dt / C);
-  const float norm = fmaxf(fabsf(2.0f * d - g) + 2.0f * d, fabsf(d - g) + d);
+  const float ad = fabsf(d);
+  const float norm = fmaxf(fabsf(2.0f * d - g) + 2.0f * ad, fabsf(d - g) + ad);
   bool ok = norm <= 0.5f && R >= 0x1p-20f && R <= 0x1p20f;
   ok &= fabsf(vin) <= 0x1p40f && fabsf(vout) <= 0x1p40f;
 #pragma unroll

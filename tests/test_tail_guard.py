@@ -1,5 +1,6 @@
 """The bound behind the kernels' tail_stays_finite (ckt_kernels.hip): a wire
-whose step contracts (infinity-norm of the substituted update at most 1/2),
+whose step contracts (infinity-norm of the substituted update at most 1/2,
+negative capacitances included),
 with 2^-20 <= R <= 2^20 and every starting current and voltage (node
 voltages included) at most 2^40 in magnitude, keeps every intermediate value
 of the reference's step (circuit.cc:387-401) below 2^104, so its last steps
@@ -21,7 +22,8 @@ DT = f32(1e-6)
 def guard(I, V, vin, vout, L, R, C):
     g = (L / DT) / R
     d = (f32(1) / R) * (DT / C)
-    norm = np.maximum(np.abs(f32(2) * d - g) + f32(2) * d, np.abs(d - g) + d)
+    ad = np.abs(d)
+    norm = np.maximum(np.abs(f32(2) * d - g) + f32(2) * ad, np.abs(d - g) + ad)
     ok = (norm <= 0.5) & (R >= 2.0**-20) & (R <= 2.0**20)
     ok &= (np.abs(vin) <= 2.0**40) & (np.abs(vout) <= 2.0**40)
     ok &= (np.abs(I) <= 2.0**40).all(1) & (np.abs(V) <= 2.0**40).all(1)
@@ -33,7 +35,7 @@ def draw(rng, n):
     # norm <= 1/2 needs g <= 1 and d <= 1/4: draw g and d up to those and
     # derive L and C from them.
     g = rng.uniform(0, 1, n).astype(f32) * (rng.random(n) < 0.7)
-    d = (f32(2.0) ** rng.uniform(-40, -2, n)).astype(f32)
+    d = (f32(2.0) ** rng.uniform(-40, -2, n) * rng.choice([-1, 1], n, p=[0.1, 0.9])).astype(f32)
     L = (g * DT * R).astype(f32)
     C = (DT / (d * R)).astype(f32)
     mag = f32(2.0) ** rng.uniform(0, 40, (n, 1)).astype(f32)
