@@ -18,10 +18,13 @@
  *     calls (suffix _gpu) wait for completion before returning, as the
  *     reference's host launchers do (circuit_gpu.cu:204,303,367).
  *   - A stream argument is a hipStream_t passed as void* (NULL = null stream).
- *   - Threading: a context, batch or server handle is used by one host thread
- *     at a time (as the reference runs a GPU's tasks on one worker thread,
- *     lowlevel_gpu.cc:117-160); distinct handles are independent and may be
- *     used concurrently.  The accessor-form calls and the generator are
+ *   - Threading: calls on one context or batch handle may come from several
+ *     host threads; the handle's mutex runs them one at a time (the reference
+ *     runs a GPU's tasks one at a time on one worker thread,
+ *     lowlevel_gpu.cc:117-160).  A server's submit, poll, wait and cancel may
+ *     be called from any thread while its scheduler runs.  Distinct handles are
+ *     independent and may be used concurrently.  Destroying a handle must not
+ *     overlap other calls on it.  The accessor-form calls and the generator are
  *     re-entrant.
  */
 #ifndef CIRCUIT_ABI_H

@@ -7,6 +7,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -15,13 +16,27 @@
 
 using namespace ckt;
 
+// Calls on one batch from several host threads are serialised by its mutex.
 struct ckt_batch {
   Engine eng;
   ckt_batch_config_t applied{};
   bool have_applied = false;
+  std::recursive_mutex mu;
 };
 
 namespace {
+
+// Holds the batch's mutex for the rest of an entry point (nothing for a null
+// batch, which the entry point then rejects).
+class BatchLock {
+ public:
+  explicit BatchLock(ckt_batch_t* b) {
+    if (b) lk_ = std::unique_lock<std::recursive_mutex>(b->mu);
+  }
+
+ private:
+  std::unique_lock<std::recursive_mutex> lk_;
+};
 
 double now_s() {
   struct timespec t;
@@ -144,12 +159,14 @@ int ckt_batch_destroy(ckt_batch_t* b) {
 }
 
 int ckt_batch_set_option(ckt_batch_t* b, int option, int64_t value) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   return b->eng.set_option(option, value);
 }
 
 int ckt_batch_set_timing(ckt_batch_t* b, int enable) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   b->eng.set_timing(enable != 0);
@@ -157,6 +174,7 @@ int ckt_batch_set_timing(ckt_batch_t* b, int enable) {
 }
 
 int ckt_batch_kernel_time(ckt_batch_t* b, int task_id, double* total_ms, uint64_t* launches) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (task_id < 0 || task_id >= CKT_NUM_TASK_IDS) return fail(CKT_ERR_INVALID_ARG, "bad task id");
@@ -171,6 +189,7 @@ int ckt_footprint_bytes(uint32_t num_nodes, uint32_t num_wires, uint32_t num_pie
 
 int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int32_t num_loops,
                             uint32_t* id) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
@@ -186,6 +205,7 @@ int ckt_batch_add_generated(ckt_batch_t* b, const ckt_gen_params_t* params, int3
 
 int ckt_batch_add_generated_device(ckt_batch_t* b, const ckt_gen_params_t* params,
                                    int32_t num_loops, uint32_t* id) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
@@ -195,6 +215,7 @@ int ckt_batch_add_generated_device(ckt_batch_t* b, const ckt_gen_params_t* param
 int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,
                                  const int32_t* num_loops, uint32_t n, int32_t threads,
                                  uint32_t* ids) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (n && (!params || !num_loops || !ids)) return fail(CKT_ERR_INVALID_ARG, "null array");
@@ -234,6 +255,7 @@ int ckt_batch_add_generated_many(ckt_batch_t* b, const ckt_gen_params_t* params,
 int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,
                   const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
                   const ckt_piece_t* pieces, uint32_t num_pieces, int32_t num_loops, uint32_t* id) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
@@ -244,12 +266,14 @@ int ckt_batch_add(ckt_batch_t* b, const ckt_node_t* nodes, uint32_t num_nodes,
 }
 
 int ckt_batch_remove(ckt_batch_t* b, uint32_t id) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   return b->eng.remove(id);
 }
 
 int ckt_batch_count(ckt_batch_t* b, uint32_t* num_circuits, uint64_t* bytes_in_use) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (num_circuits) *num_circuits = (uint32_t)b->eng.count();
@@ -258,6 +282,7 @@ int ckt_batch_count(ckt_batch_t* b, uint32_t* num_circuits, uint64_t* bytes_in_u
 }
 
 int ckt_batch_loops_left(ckt_batch_t* b, uint32_t id, int32_t* loops_left) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   Circuit* c = b->eng.find(id);
@@ -267,6 +292,7 @@ int ckt_batch_loops_left(ckt_batch_t* b, uint32_t id, int32_t* loops_left) {
 }
 
 int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* active) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   const ckt_batch_config_t def = {CKT_POLICY_FAST, CKT_REDUCE_ATOMIC, 1, 0, 0, 0};
@@ -293,6 +319,7 @@ int ckt_batch_step(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t* ac
 }
 
 int ckt_batch_run(ckt_batch_t* b, const ckt_batch_config_t* config, ckt_batch_report_t* report) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   const ckt_batch_config_t def = {CKT_POLICY_FAST, CKT_REDUCE_ATOMIC, 1, 0, 0, 0};
@@ -341,6 +368,7 @@ int ckt_batch_run(ckt_batch_t* b, const ckt_batch_config_t* config, ckt_batch_re
 }
 
 int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* wires) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   return b->eng.store(id, nodes, wires);
@@ -352,6 +380,7 @@ int ckt_batch_store(ckt_batch_t* b, uint32_t id, ckt_node_t* nodes, ckt_wire_t* 
 // order) with that shape.
 int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
                        ckt_batch_config_t* best, double* best_ms) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (!best) return fail(CKT_ERR_INVALID_ARG, "null best");
@@ -394,6 +423,7 @@ int ckt_batch_autotune(ckt_batch_t* b, int32_t policy, int32_t probe_steps,
 
 int ckt_batch_probe(ckt_batch_t* b, const ckt_batch_config_t* config, int32_t probe_steps,
                     double* ms) {
+  BatchLock lk(b);
   int st = check(b);
   if (st) return st;
   if (!config || !ms) return fail(CKT_ERR_INVALID_ARG, "null argument");

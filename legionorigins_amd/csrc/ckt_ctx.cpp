@@ -2,6 +2,7 @@
 // task-variant table and the region_main loop (apps/circuit/circuit.cc:123-346).
 #include <time.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -9,13 +10,28 @@
 
 using namespace ckt;
 
+// Calls on one context from several host threads are serialised by its
+// mutex (recursive: the task-variant table calls the task entry points).
 struct ckt_ctx {
   Engine eng;
   uint32_t id = 0;
   bool loaded = false;
+  std::recursive_mutex mu;
 };
 
 namespace {
+
+// Holds the context's mutex for the rest of an entry point (nothing for a
+// null context, which the entry point then rejects).
+class CtxLock {
+ public:
+  explicit CtxLock(ckt_ctx_t* ctx) {
+    if (ctx) lk_ = std::unique_lock<std::recursive_mutex>(ctx->mu);
+  }
+
+ private:
+  std::unique_lock<std::recursive_mutex> lk_;
+};
 
 int check_ctx(ckt_ctx_t* ctx) {
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
@@ -78,6 +94,7 @@ int ckt_ctx_destroy(ckt_ctx_t* ctx) {
 void* ckt_ctx_stream(ckt_ctx_t* ctx) { return ctx ? (void*)ctx->eng.stream() : nullptr; }
 
 int ckt_ctx_synchronize(ckt_ctx_t* ctx) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   return ctx->eng.sync();
 }
@@ -85,6 +102,7 @@ int ckt_ctx_synchronize(ckt_ctx_t* ctx) {
 int ckt_ctx_load(ckt_ctx_t* ctx, const ckt_node_t* nodes, uint32_t num_nodes,
                  const ckt_wire_t* wires, uint32_t num_wires, const int32_t* locator,
                  const ckt_piece_t* pieces, uint32_t num_pieces) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   SlabImage img;
   int st = image_from_aos(img, nodes, num_nodes, wires, num_wires, locator, pieces, num_pieces);
@@ -93,6 +111,7 @@ int ckt_ctx_load(ckt_ctx_t* ctx, const ckt_node_t* nodes, uint32_t num_nodes,
 }
 
 int ckt_ctx_generate(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   SlabImage img;
   int st = image_from_gen(img, params);
@@ -101,6 +120,7 @@ int ckt_ctx_generate(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
 }
 
 int ckt_ctx_generate_device(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   int st = gen_validate(params, nullptr, nullptr, nullptr);
   if (st) return st;
@@ -114,6 +134,7 @@ int ckt_ctx_generate_device(ckt_ctx_t* ctx, const ckt_gen_params_t* params) {
 }
 
 int ckt_ctx_store(ckt_ctx_t* ctx, ckt_node_t* nodes, ckt_wire_t* wires) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   return ctx->eng.store(ctx->id, nodes, wires);
@@ -121,12 +142,14 @@ int ckt_ctx_store(ckt_ctx_t* ctx, ckt_node_t* nodes, ckt_wire_t* wires) {
 
 int ckt_ctx_store_topology(ckt_ctx_t* ctx, int32_t* locator, uint32_t* csr_row,
                            uint32_t* csr_ent) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   return ctx->eng.store_topology(ctx->id, locator, csr_row, csr_ent);
 }
 
 int ckt_ctx_sizes(ckt_ctx_t* ctx, uint32_t* num_nodes, uint32_t* num_wires, uint32_t* num_pieces) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   const Circuit* c = ctx->eng.find(ctx->id);
@@ -137,6 +160,7 @@ int ckt_ctx_sizes(ckt_ctx_t* ctx, uint32_t* num_nodes, uint32_t* num_wires, uint
 }
 
 int ckt_ctx_pieces(ckt_ctx_t* ctx, ckt_piece_t* pieces) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   if (!pieces) return fail(CKT_ERR_INVALID_ARG, "null piece array");
@@ -146,6 +170,7 @@ int ckt_ctx_pieces(ckt_ctx_t* ctx, ckt_piece_t* pieces) {
 }
 
 int ckt_calc_new_currents(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, int policy) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   std::vector<PieceRef> refs;
@@ -154,6 +179,7 @@ int ckt_calc_new_currents(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n,
 }
 
 int ckt_distribute_charge(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, int mode) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   std::vector<PieceRef> refs;
@@ -162,6 +188,7 @@ int ckt_distribute_charge(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n,
 }
 
 int ckt_update_voltages(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   std::vector<PieceRef> refs;
@@ -171,6 +198,7 @@ int ckt_update_voltages(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n) {
 
 int ckt_sanity_check_wires(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n,
                            uint64_t* num_bad) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   std::vector<PieceRef> refs;
@@ -180,6 +208,7 @@ int ckt_sanity_check_wires(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n
 
 int ckt_execute_index_space(ckt_ctx_t* ctx, int task_id, const ckt_piece_t* pieces, uint32_t n,
                             const ckt_task_config_t* config) {
+  CtxLock lk(ctx);
   const ckt_task_config_t def = {CKT_POLICY_EXACT, CKT_REDUCE_ATOMIC};
   const ckt_task_config_t* cfg = config ? config : &def;
   switch (task_id) {
@@ -209,6 +238,7 @@ int ckt_execute_index_space(ckt_ctx_t* ctx, int task_id, const ckt_piece_t* piec
 
 int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config,
                   ckt_loop_report_t* report) {
+  CtxLock lk(ctx);
   int st = check_ctx(ctx);
   if (st) return st;
   if (num_loops < 0) return fail(CKT_ERR_INVALID_ARG, "negative loop count");
@@ -290,23 +320,27 @@ int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config
 }
 
 int ckt_ctx_set_option(ckt_ctx_t* ctx, int option, int64_t value) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   return ctx->eng.set_option(option, value);
 }
 
 int ckt_ctx_set_timing(ckt_ctx_t* ctx, int enable) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   ctx->eng.set_timing(enable != 0);
   return CKT_OK;
 }
 
 int ckt_ctx_kernel_time(ckt_ctx_t* ctx, int task_id, double* total_ms, uint64_t* launches) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   if (task_id < 0 || task_id >= CKT_NUM_TASK_IDS) return fail(CKT_ERR_INVALID_ARG, "bad task id");
   return ctx->eng.kernel_time(task_id, total_ms, launches);
 }
 
 int ckt_ctx_reset_timing(ckt_ctx_t* ctx) {
+  CtxLock lk(ctx);
   if (!ctx) return fail(CKT_ERR_INVALID_ARG, "null context");
   ctx->eng.reset_timing();
   return CKT_OK;

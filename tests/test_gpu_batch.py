@@ -227,3 +227,38 @@ def test_batch_small_circuits_equal_oracle(C, O):
         assert nodes.tobytes() == ref.nodes.tobytes()
         assert wires.tobytes() == ref.wires.tobytes()
     b.close()
+
+
+def test_batch_steps_from_several_threads(C, O):
+    # Two host threads each take two batch steps at once; the batch's mutex
+    # runs the four steps one at a time, so every circuit runs 4 loops,
+    # bit-identical (EXACT + CSR) to the oracle's 4-loop run.
+    import threading
+    shapes = [(P, npp, wpp, pct, seed) for (P, npp, wpp, pct, seed, _) in SHAPES[:4]]
+    b = C.Batch(0)
+    ids = [b.add_generated(C.GenParams(P, npp, wpp, pct, seed, C.PROFILE_STABLE), 4)
+           for (P, npp, wpp, pct, seed) in shapes]
+    cfg = C.Batch.config(policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+    errors = []
+
+    def steps():
+        try:
+            for _ in range(2):
+                b.step(cfg)
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=steps) for _ in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for cid, (P, npp, wpp, pct, seed) in zip(ids, shapes):
+        assert b.loops_left(cid) == 0
+        nodes, wires = b.store(cid, P * npp, P * wpp)
+        ref = O.generate(P, npp, wpp, pct, seed, O.PROFILE_STABLE)
+        O.run(ref, 4)
+        assert nodes.tobytes() == ref.nodes.tobytes()
+        assert wires.tobytes() == ref.wires.tobytes()
+    b.close()

@@ -366,3 +366,44 @@ def test_load_validation(C):
             ctx.load(bad)
         ctx.load(host)
         assert ctx.sizes() == (64, 128, 2)
+
+
+def test_context_calls_from_several_threads(C, O):
+    # Calls on one context from several host threads run one at a time: four
+    # threads each run 2 loops of EXACT + CSR (one call each, holding the
+    # context for both loops) while a fifth keeps reading the piece table and
+    # sizes; the circuit ends after 8 loops, bit-identical to the oracle's.
+    import threading
+    s = SMALL
+    errors = []
+    with C.Context(0) as ctx:
+        ctx.generate(C.GenParams(s["P"], s["npp"], s["wpp"], s["pct"], s["seed"], C.PROFILE_STABLE))
+        done = threading.Event()
+
+        def loops():
+            try:
+                ctx.run_loops(2, policy=C.POLICY_EXACT, mode=C.REDUCE_CSR)
+            except Exception as e:  # noqa: BLE001 -- reported below
+                errors.append(e)
+
+        def reader():
+            try:
+                while not done.is_set():
+                    assert len(ctx.pieces()) == s["P"]
+            except Exception as e:  # noqa: BLE001
+                errors.append(e)
+
+        r = threading.Thread(target=reader)
+        r.start()
+        ts = [threading.Thread(target=loops) for _ in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        done.set()
+        r.join()
+        assert not errors, errors
+        nodes, wires = ctx.store()
+    ref = run_oracle(O, s["P"], s["npp"], s["wpp"], s["pct"], s["seed"], O.PROFILE_STABLE, 8)
+    assert nodes.tobytes() == ref.nodes.tobytes()
+    assert wires.tobytes() == ref.wires.tobytes()
