@@ -17,8 +17,9 @@ oracle's restatement of the reference task bodies, one loop over whole pieces,
 one piece per host thread) and `detail`: per-policy and per-reduction timings,
 the batched workload (512 circuits in shared launches against sequential
 launches), the served workload (Poisson 200 requests/s for 60 s: p50/p99
-latency and throughput against one request at a time with plain launches) and
-the FP64 error of each workload.
+latency and throughput against one request at a time with plain launches),
+the FP64 error of each workload, and the rate of each calling form (the
+accessor-form drop-ins per piece and per index space, the resident context).
 """
 from __future__ import annotations
 
@@ -36,7 +37,7 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (public spec)
 FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
 # FP32 flops each policy's step executes (an FMA counts 2), from its loop
 # body's instruction mix, which the SQ_INSTS_VALU_FLOPS_FP32 counter confirms
-# per launch (profiles/calc_new_currents_4M_wires_*_counters.md):
+# per launch (profiles/calc_new_currents_4M_wires_policy_counters.md):
 #   EXACT  the reference's 67 FP32 operations; its 29 divisions are FP64
 #          multiplies by a per-wire reciprocal (29 FP64 flops, not counted here)
 #   RECIP  10 x (subtract + 2 FMA) + 9 x (subtract + FMA) = 77
@@ -70,6 +71,8 @@ def parse():
     ap.add_argument("--skip-batched", action="store_true", help="detail without the batched run")
     ap.add_argument("--skip-served", action="store_true", help="detail without the served run")
     ap.add_argument("--skip-fp64", action="store_true", help="detail without the FP64 errors")
+    ap.add_argument("--skip-accessor", action="store_true",
+                    help="detail without the calling-form rates (accessor-form drop-ins)")
     ap.add_argument("--batched", action="store_true",
                     help="batched workload: 512 circuits in shared launches vs sequential")
     ap.add_argument("--served", action="store_true",
@@ -495,7 +498,7 @@ def run_dsc_sweep(C, args):
                       "rows": rows}))
 
 
-def run_accessor(C, args):
+def accessor_measure(C, policy_name, loops=3):
     """The three tasks through the accessor-form entry points -- what the
     reference's GPU task wrappers call (circuit.cc:475-550) -- on the 64-piece
     circuit (4.19 M wires) in the reference layouts (128-byte wires, 16-byte
@@ -510,7 +513,7 @@ def run_accessor(C, args):
     import numpy as np
     wl = dict(WORKLOADS["64x16384x65536"])
     wl.pop("loops")
-    policy = policy_of(C, args.policy)
+    policy = policy_of(C, policy_name)
     params = C.GenParams(**wl, profile=C.PROFILE_REFERENCE)
     host = C.generate(params)
     N, W = len(host.nodes), len(host.wires)
@@ -542,7 +545,6 @@ def run_accessor(C, args):
         C.fold_apply_gpu(nodes, gfold, 0, N)
         C.update_voltages_index_gpu(pieces, nodes, nodes, loc)
 
-    loops = 3
     rows = {}
     for name, fn in (("accessor_form_per_piece_sync", loop),
                      ("accessor_form_index_space", loop_index)):
@@ -564,8 +566,12 @@ def run_accessor(C, args):
         loops, policy=policy, mode=C.REDUCE_ATOMIC)["wire_updates_per_s"]
     ctx.close()
     log(f"[accessor] {rows}")
-    print(json.dumps({"metric": "wire-updates/s at 64x16384x65536 by calling form", "unit": "wire-updates/s",
-                      "policy": args.policy, "loops": loops, "rows": rows}))
+    return {"unit": "wire-updates/s", "policy": policy_name, "loops": loops, "rows": rows}
+
+
+def run_accessor(C, args):
+    d = accessor_measure(C, args.policy)
+    print(json.dumps({"metric": "wire-updates/s at 64x16384x65536 by calling form", **d}))
 
 
 def run_gen(C, args):
@@ -772,7 +778,8 @@ def main():
         ("served", not args.skip_served,
          lambda: served_measure(C, args.policy, args.serve_rate, args.serve_seconds,
                                 args.serve_cap_wires, args.serve_timeout, args.compare_seconds)),
-        ("fp64_error", not args.skip_fp64, lambda: fp64_errors(C, args.policy))]
+        ("fp64_error", not args.skip_fp64, lambda: fp64_errors(C, args.policy)),
+        ("calling_forms", not args.skip_accessor, lambda: accessor_measure(C, args.policy))]
     for key, on, fn in extra:
         if not on:
             continue
