@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -51,6 +52,33 @@ struct IndexLayout {
   explicit IndexLayout(uint32_t n)
       : first_block(16), pieces(index_pieces_offset(n)), bytes(index_bytes(n)) {}
 };
+
+// The pieces of one call must not share wires (or nodes, for a task over
+// nodes): the reference's pieces partition the circuit, and a wire in two
+// pieces of one launch would be relaxed twice at once.  `order` is scratch
+// for n indices (the caller's, so the allocation-free entry point stays
+// allocation-free).
+static int check_disjoint(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, uint32_t* order,
+                          const char* what) {
+  auto first = [&](uint32_t i) -> uint64_t {
+    return over_nodes ? pieces[i].first_node : pieces[i].first_wire;
+  };
+  auto count = [&](uint32_t i) -> uint64_t {
+    return over_nodes ? pieces[i].num_nodes : pieces[i].num_wires;
+  };
+  uint32_t m = 0;
+  for (uint32_t i = 0; i < n; i++)
+    if (count(i)) order[m++] = i;
+  std::sort(order, order + m, [&](uint32_t a, uint32_t b) { return first(a) < first(b); });
+  for (uint32_t k = 1; k < m; k++) {
+    const uint32_t a = order[k - 1], b = order[k];
+    if (first(a) + count(a) > first(b))
+      return fail(CKT_ERR_INVALID_ARG,
+                  std::string(what) + ": pieces " + std::to_string(a) + " and " +
+                      std::to_string(b) + " share " + (over_nodes ? "nodes" : "wires"));
+  }
+  return CKT_OK;
+}
 
 // Host image of the scratch's head: the zeroed counter (image[0..3]) and
 // first_block[] over each piece's wires or nodes (image[4..4+n]); n + 5 words.
@@ -107,9 +135,10 @@ static int run_indexed(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, u
 template <class Launch>
 static int run_indexed_alloc(const ckt_piece_t* pieces, uint32_t n, bool over_nodes, hipStream_t s,
                              const char* what, uint64_t* nbad, Launch launch) {
+  std::vector<uint32_t> image((size_t)n + 5);
+  if (int st = check_disjoint(pieces, n, over_nodes, image.data(), what)) return st;
   unsigned char* dev = nullptr;
   CKT_HIP(hipMallocAsync((void**)&dev, IndexLayout(n).bytes, s));
-  std::vector<uint32_t> image((size_t)n + 5);
   const int st = run_indexed(pieces, n, over_nodes, dev, image.data(), s, what, nbad, launch);
   const hipError_t e = hipFreeAsync(dev, s);
   if (st) return st;
@@ -269,10 +298,11 @@ int ckt_calc_new_currents_index_gpu_noalloc(const ckt_piece_t* pieces, uint32_t 
                 "ckt_calc_new_currents_index_gpu_noalloc: " + std::to_string(num_pieces) +
                     " pieces, more than CKT_NOALLOC_MAX_PIECES (" +
                     std::to_string(CKT_NOALLOC_MAX_PIECES) + ")");
+  uint32_t image[CKT_NOALLOC_MAX_PIECES + 5];
+  if (int st = check_disjoint(pieces, num_pieces, false, image, what)) return st;
   unsigned char* dev = nullptr;
   if (int st = noalloc_scratch(&dev)) return st;
   hipStream_t s = (hipStream_t)stream;
-  uint32_t image[CKT_NOALLOC_MAX_PIECES + 5];
   uint64_t nbad = 0;
   int st = run_indexed(pieces, num_pieces, false, dev, image, s, what, &nbad,
                        [&](const PieceIndex& ix, uint32_t blocks, unsigned long long* bad) {

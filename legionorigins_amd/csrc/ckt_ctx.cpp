@@ -2,6 +2,7 @@
 // task-variant table and the region_main loop (apps/circuit/circuit.cc:123-346).
 #include <time.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -41,6 +42,9 @@ int check_ctx(ckt_ctx_t* ctx) {
   return CKT_OK;
 }
 
+// The loaded pieces named by `pieces`, each at most once (the loaded pieces
+// partition the circuit, so a piece named twice is the only way one launch
+// could process a wire or node twice).
 int refs_from(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, std::vector<PieceRef>& refs) {
   if (n && !pieces) return fail(CKT_ERR_INVALID_ARG, "null piece array");
   refs.resize(n);
@@ -48,6 +52,11 @@ int refs_from(ckt_ctx_t* ctx, const ckt_piece_t* pieces, uint32_t n, std::vector
     int st = ctx->eng.ref_for(0, pieces[i], &refs[i]);
     if (st) return st;
   }
+  std::vector<uint32_t> seen(n);
+  for (uint32_t i = 0; i < n; i++) seen[i] = refs[i].piece;
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(CKT_ERR_INVALID_ARG, "a piece is named twice in one call");
   return CKT_OK;
 }
 

@@ -92,3 +92,28 @@ def test_no_device_is_an_error_not_an_abort(C):
     assert L.ckt_calc_new_currents_gpu(None, None, None, None, None, 0, None) == 1
     assert L.ckt_ctx_synchronize(None) == 1
     assert L.ckt_run_loops(None, 1, None, None) == 1
+
+
+def test_index_form_rejects_overlapping_pieces(C):
+    # The pieces of one index-space call must not share wires (calc_new_currents,
+    # distribute_charge) or nodes (update_voltages): the check runs on the host
+    # before any device call, so it needs no GPU (the instance pointers are
+    # never dereferenced).
+    L = C.lib()
+    dummy = ctypes.c_void_p(256)
+    pieces = np.zeros(3, C.PIECE)
+    pieces["first_wire"] = [0, 4096, 4000]       # third overlaps the first
+    pieces["num_wires"] = [4096, 4096, 200]
+    pieces["first_node"] = [0, 1024, 2048]
+    pieces["num_nodes"] = [1024, 1024, 1024]
+    p = pieces.ctypes.data_as(ctypes.c_void_p)
+    assert L.ckt_calc_new_currents_index_gpu(p, 3, dummy, dummy, dummy, dummy, 0, None) == 1
+    assert "share wires" in L.ckt_last_error().decode()
+    assert L.ckt_calc_new_currents_index_gpu_noalloc(p, 3, dummy, dummy, dummy, dummy, 0, None) == 1
+    assert L.ckt_distribute_charge_index_gpu(p, 3, dummy, dummy, dummy, dummy, 0, None) == 1
+    nodes = pieces.copy()
+    nodes["first_wire"] = [0, 4096, 8192]
+    nodes["first_node"] = [0, 1024, 1500]        # third overlaps the second's nodes
+    q = nodes.ctypes.data_as(ctypes.c_void_p)
+    assert L.ckt_update_voltages_index_gpu(q, 3, dummy, dummy, dummy, 0, None) == 1
+    assert "share nodes" in L.ckt_last_error().decode()
