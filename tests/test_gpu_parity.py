@@ -160,6 +160,14 @@ def test_per_piece_calls_and_errors(C, O):
         assert e.value.status == 2
         with pytest.raises(C.CircuitError):
             ctx.calc_new_currents(pieces, policy=7)
+        # a piece named twice in one call would be processed twice at once
+        twice = np.concatenate([pieces, pieces[1:2]])
+        for call in (lambda: ctx.calc_new_currents(twice), lambda: ctx.update_voltages(twice),
+                     lambda: ctx.distribute_charge(twice, C.REDUCE_ATOMIC)):
+            with pytest.raises(C.CircuitError) as e:
+                call()
+            assert e.value.status == 1
+        assert ctx.store()[0].tobytes() == nodes.tobytes()
     ref = O.generate(4, 128, 512, 80, 22, O.PROFILE_STABLE)
     O.run(ref, 1)
     assert_bit_identical(nodes["voltage"], ref.nodes["voltage"], "node voltage")
