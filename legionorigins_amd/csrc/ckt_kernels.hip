@@ -48,11 +48,36 @@ __device__ __forceinline__ float div_fixed(float a, double rb, float b) {
   return q;
 }
 
+// div_fixed over N dividends by one divisor, with one guard for the group:
+// the rare path runs when the smallest |quotient| is below 2^-126 and then
+// takes the IEEE division for exactly the quotients div_fixed would (zero or
+// subnormal, nonzero dividend), so the results are div_fixed's, bit for bit.
+// fminf ignores NaN quotients, which div_fixed's compare also lets through.
+// Four 3-input minimums (absolute values are free source modifiers) and one
+// compare replace ten compares.
+template <int N>
+__device__ __forceinline__ void div_fixed_group(float (&q)[N], const float (&a)[N], double rb,
+                                                float b) {
+  float m = __builtin_huge_valf();
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    q[i] = (float)((double)a[i] * rb);
+    m = fminf(m, fabsf(q[i]));
+  }
+  if (__builtin_expect(m < 0x1p-126f, 0)) {
+    asm volatile("");
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (fabsf(q[i]) < 0x1p-126f && a[i] != 0.f) q[i] = a[i] / b;
+  }
+}
+
 // The reference's step in its order of operations (circuit_gpu.cu:114-129),
 // `steps` times, on the running currents ni and voltages nv (nv[0], nv[kSeg]
 // the node voltages), with I0 and V0 the wire's values at the start of the
 // loop; writes the result to I0 and V0.  Every division is correctly rounded
-// (div_fixed).
+// (div_fixed_group: the ten divisions by dt, then the ten by R, then the nine
+// by C, each group with one guard).
 __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[kSeg + 1],
                                                 float (&I0)[kSeg], float (&V0)[kSeg - 1], float L,
                                                 float R, float C, int steps) {
@@ -60,12 +85,18 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
   constexpr double rdt = 1.0 / (double)CKT_DT_F;
   const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
   for (int j = 0; j < steps; j++) {
+    float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
 #pragma unroll
-    for (int i = 0; i < kSeg; i++)
-      ni[i] = div_fixed((nv[i + 1] - nv[i]) - div_fixed(L * (ni[i] - I0[i]), rdt, dt), rR, R);
+    for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
+    div_fixed_group(q, t, rdt, dt);
 #pragma unroll
-    for (int i = 0; i < kSeg - 1; i++)
-      nv[i + 1] = V0[i] + div_fixed(dt * (ni[i] - ni[i + 1]), rC, C);
+    for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
+    div_fixed_group(ni, t, rR, R);
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
+    div_fixed_group(w, u, rC, C);
+#pragma unroll
+    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
   }
 #pragma unroll
   for (int i = 0; i < kSeg; i++) I0[i] = ni[i];
