@@ -57,7 +57,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed loops")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed loops")
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed loops (the clock settles over the first few launches)")
     ap.add_argument("--policy", default="scaled", choices=["exact", "recip", "fast", "scaled"])
     ap.add_argument("--mode", default="atomic", choices=["atomic", "csr", "kahan64"])
     ap.add_argument("--workload", default="64x16384x65536",
