@@ -126,7 +126,8 @@ def time_loops(C, ctx, loops, policy, mode):
 
 def host_threads():
     """Every core in this process's affinity mask, capped by OMP_NUM_THREADS
-    where the host sets it (a GPU box's share of its CPUs)."""
+    where the environment sets it (the share of the host's CPUs given to
+    this process)."""
     n = len(os.sched_getaffinity(0))
     cap = os.environ.get("OMP_NUM_THREADS")
     return max(1, min(n, int(cap))) if cap and cap.isdigit() else n

@@ -405,7 +405,7 @@ int ckt_debug_bounds_violations(int device, uint64_t* count) {
   CKT_HIP(hipDeviceSynchronize());
   const hipError_t e = read_bounds_violations(count);
   if (e == hipErrorNotSupported)
-    return fail(CKT_ERR_UNSUPPORTED, "this build has no bounds checks (CKT_BOUNDS_CHECKS)");
+    return fail(CKT_ERR_UNSUPPORTED, "the library was built without bounds checks (CKT_BOUNDS_CHECKS)");
   if (e != hipSuccess) return hip_fail(e, "bounds-check counter");
   return CKT_OK;
 }
