@@ -38,12 +38,13 @@ FLOPS_PER_WIRE_STEP = 96   # (10*6 + 9*4), apps/circuit/circuit.cc:324
 # FP32 flops each policy's step executes (an FMA counts 2), from its loop
 # body's instruction mix, which the SQ_INSTS_VALU_FLOPS_FP32 counter confirms
 # per launch (profiles/calc_new_currents_4M_wires_policy_counters.md):
-#   EXACT  the reference's 67 FP32 operations; its 29 divisions are FP64
-#          multiplies by a per-wire reciprocal (29 FP64 flops, not counted here)
+#   EXACT  the reference's 67 FP32 operations (48 adds, 19 multiplies), and
+#          its 29 correctly rounded divisions as two FMAs each (a split
+#          reciprocal of the divisor): 67 + 29 x 4 = 183
 #   RECIP  10 x (subtract + 2 FMA) + 9 x (subtract + FMA) = 77
 #   FAST   20 FMA + 8 add = 48
 #   SCALED 10 FMA + 6 add, plus 10 multiplies per 4 steps = 28.5
-EXECUTED_FLOPS = {"exact": 67, "recip": 77, "fast": 48, "scaled": 28.5}
+EXECUTED_FLOPS = {"exact": 183, "recip": 77, "fast": 48, "scaled": 28.5}
 # baseline.cu on one Tesla M2090, 48 x 2500 x 16384, 10 loops, 181.033 s
 # (apps/circuit/keeneland_results/baseline_48_1_1_1.stdio).
 BASELINE_WIRE_UPDATES_PER_S = 48 * 16384 * 10 / 181.033

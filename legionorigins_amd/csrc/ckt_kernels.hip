@@ -11,6 +11,7 @@
 // generated (-packed-fp32-ops).
 #include <hip/hip_runtime.h>
 
+#include "ckt_divsplit.h"
 #include "ckt_index.h"
 #include "ckt_internal.h"
 
@@ -72,31 +73,97 @@ __device__ __forceinline__ void div_fixed_group(float (&q)[N], const float (&a)[
   }
 }
 
+// A divisor with its split for div_split_group: (zh, zl) from make_div_split,
+// zh = zl = 0 if it has none (every quotient then takes div_fixed).
+struct SplitDivisor {
+  float b, zh, zl;
+  __device__ __forceinline__ explicit SplitDivisor(float divisor) : b(divisor) {
+    make_div_split(divisor, &zh, &zl);
+  }
+};
+
+// Correctly rounded a[i] / b for N dividends, FP32 only: q = RN(a zh - u),
+// u = RN(-a zl + 0), two fmas each (div_split_q, ckt_divsplit.h); zero
+// dividends get the zero of the right sign.  One minimum over |q| and one
+// maximum over |u| guard the group (fminf and fmaxf ignore NaN, and a NaN
+// dividend gives a NaN quotient on every path); the minimum starts at |zh|,
+// which is at least 2^-61 for a divisor with a split and 0 for one without.
+// The guard also fires on zero quotients, the common case once a wire has
+// settled; a second test inside it looks for what the split does not cover,
+// and then the whole group goes to div_fixed: a divisor without a split, an
+// infinite u (infinite dividends, quotients beyond about 2^150), or a nonzero
+// dividend below |b| 2^-98 (a quotient that may be below kDivSplitMin; the
+// integer keys of div_split_key put zeros above every bound).
+template <int N>
+__device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[N],
+                                                const SplitDivisor& d) {
+  float m = fabsf(d.zh), M = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float u;
+    q[i] = div_split_q(a[i], d.zh, d.zl, &u);
+    m = fminf(m, fabsf(q[i]));
+    M = fmaxf(M, fabsf(u));
+  }
+  if (__builtin_expect(m < kDivSplitMin || M > __FLT_MAX__, 0)) {
+    asm volatile("");
+    uint32_t lo = 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < N; i++) lo = min(lo, div_split_key(a[i]));
+    if (lo < div_split_key_bound(d.b) || M > __FLT_MAX__ || d.zh == 0.f) {
+      const double rb = 1.0 / (double)d.b;
+#pragma unroll
+      for (int i = 0; i < N; i++) q[i] = div_fixed(a[i], rb, d.b);
+    }
+  }
+}
+
 // The reference's step in its order of operations (circuit_gpu.cu:114-129),
 // `steps` times, on the running currents ni and voltages nv (nv[0], nv[kSeg]
 // the node voltages), with I0 and V0 the wire's values at the start of the
-// loop; writes the result to I0 and V0.  Every division is correctly rounded
-// (div_fixed_group: the ten divisions by dt, then the ten by R, then the nine
-// by C, each group with one guard).
+// loop; writes the result to I0 and V0.  Every division is correctly rounded:
+// the ten by dt, then the ten by R, then the nine by C, each group with one
+// guard; by the FP32 split (div_split_group, after a per-wire check of the
+// divisors) when kSplit, else by the FP64 reciprocal (div_fixed_group, no
+// set-up: the short tails of the other policies).
+template <bool kSplit>
 __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[kSeg + 1],
                                                 float (&I0)[kSeg], float (&V0)[kSeg - 1], float L,
                                                 float R, float C, int steps) {
   const float dt = CKT_DT_F;
-  constexpr double rdt = 1.0 / (double)CKT_DT_F;
-  const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
-  for (int j = 0; j < steps; j++) {
-    float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
+  if constexpr (kSplit) {
+    const SplitDivisor sdt(dt), sR(R), sC(C);
+    for (int j = 0; j < steps; j++) {
+      float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
 #pragma unroll
-    for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
-    div_fixed_group(q, t, rdt, dt);
+      for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
+      div_split_group(q, t, sdt);
 #pragma unroll
-    for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
-    div_fixed_group(ni, t, rR, R);
+      for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
+      div_split_group(ni, t, sR);
 #pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
-    div_fixed_group(w, u, rC, C);
+      for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
+      div_split_group(w, u, sC);
 #pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
+      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
+    }
+  } else {
+    constexpr double rdt = 1.0 / (double)CKT_DT_F;
+    const double rR = 1.0 / (double)R, rC = 1.0 / (double)C;
+    for (int j = 0; j < steps; j++) {
+      float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
+      div_fixed_group(q, t, rdt, dt);
+#pragma unroll
+      for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
+      div_fixed_group(ni, t, rR, R);
+#pragma unroll
+      for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
+      div_fixed_group(w, u, rC, C);
+#pragma unroll
+      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < kSeg; i++) I0[i] = ni[i];
@@ -116,7 +183,7 @@ struct Relax<CKT_POLICY_EXACT> {
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
     nv[kSeg] = vout;
-    reference_steps(ni, nv, I, V, L, R, C, steps);
+    reference_steps<true>(ni, nv, I, V, L, R, C, steps);
   }
 };
 
@@ -286,7 +353,7 @@ __device__ __forceinline__ void finish_in_reference_order(float (&I)[kSeg], floa
 #pragma unroll
   for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i] + div_fixed(dt * (x[i] - x[i + 1]), rC, C);
   nv[kSeg] = vout;
-  reference_steps(x, nv, I, V, L, R, C, kTail);
+  reference_steps<false>(x, nv, I, V, L, R, C, kTail);
 }
 
 // Per-wire constants hoisted, FMAs in the loop, voltages kept every step:

@@ -1,6 +1,16 @@
-"""The property the EXACT policy's division rests on, checked on the CPU with
-IEEE float32/float64 arithmetic (numpy): for float a and b, with
-rb = RN64(1/b),
+"""The two correctly rounded divisions of the EXACT step, checked on the CPU.
+
+The FP32 split division (legionorigins_amd/csrc/ckt_divsplit.h), the one the
+step loop uses: tests/native/div_split.cpp, built for the host from the same
+header the kernels use, checks every dividend significand against IEEE
+division for the step's divisors, edge divisors and random ones (including
+divisors that need the moved split), random operand pairs over the whole
+exponent range, the special dividends the kernel's guard has to catch, and
+that at least 99.8% of divisors get a verified split.
+
+The FP64 reciprocal, which takes what the split does not cover and the short
+reference-order tails of the other policies, checked with IEEE float32/float64
+arithmetic (numpy): for float a and b, with rb = RN64(1/b),
 
     RN32(RN64(a * rb)) == RN32(a / b)   whenever a / b is a normal float.
 
@@ -12,7 +22,13 @@ division; this test checks the normal range, with random operands, with the
 divisors the circuit uses (dt, R in [1, 11], C in [0.001, 0.011]) and with
 dividends placed within two ulps of b*m for random midpoints m.
 """
+import os
+import subprocess
+import tempfile
+
 import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 FLT_MIN = np.float32(2.0 ** -126)
 
@@ -63,3 +79,20 @@ def test_dividends_next_to_midpoints():
         for _ in range(abs(d)):
             a = np.nextafter(a, np.float32(np.inf) if d > 0 else np.float32(-np.inf))
         check(a, b)
+
+
+def test_split_division_exhaustive():
+    from legionorigins_amd import build as B
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "div_split")
+        cmd = [B.hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "--offload-arch=gfx950", "-x", "hip",
+               "-I" + os.path.join(ROOT, "legionorigins_amd", "csrc"),
+               os.path.join(ROOT, "tests", "native", "div_split.cpp"), "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        env = dict(os.environ)
+        env.pop("LD_PRELOAD", None)
+        r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
+        print(r.stdout)
+        assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+        assert "split division: ok" in r.stdout
