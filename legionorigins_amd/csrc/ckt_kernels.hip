@@ -73,6 +73,15 @@ __device__ __forceinline__ void div_fixed_group(float (&q)[N], const float (&a)[
   }
 }
 
+// Whether `p` holds on every active lane of the wave.  Branches on it are
+// uniform: EXACT picks its step variant by it, and the policies skip the tail
+// only when the whole wave may (skipping changes the loop's trip count, and a
+// trip count that differs between lanes turns the counter of the step loop
+// into a per-lane register with an exec-mask update every iteration: +0.5
+// VALU per step for SCALED, measured 2.6% more instructions and 6% more
+// cycles; running the tail on a wire that could skip it is also correct).
+__device__ __forceinline__ bool wave_all(bool p) { return __all(p); }
+
 // A divisor with its split for div_split_group: (zh, zl) from make_div_split,
 // zh = zl = 0 if it has none (every quotient then takes div_fixed).
 struct SplitDivisor {
@@ -84,17 +93,18 @@ struct SplitDivisor {
 
 // Correctly rounded a[i] / b for N dividends, FP32 only: q = RN(a zh - u),
 // u = RN(-a zl + 0), two fmas each (div_split_q, ckt_divsplit.h); zero
-// dividends get the zero of the right sign.  One minimum over |q| and one
-// maximum over |u| guard the group (fminf and fmaxf ignore NaN, and a NaN
-// dividend gives a NaN quotient on every path); the minimum starts at |zh|,
-// which is at least 2^-61 for a divisor with a split and 0 for one without.
-// The guard also fires on zero quotients, the common case once a wire has
-// settled; a second test inside it looks for what the split does not cover,
-// and then the whole group goes to div_fixed: a divisor without a split, an
-// infinite u (infinite dividends, quotients beyond about 2^150), or a nonzero
-// dividend below |b| 2^-98 (a quotient that may be below kDivSplitMin; the
-// integer keys of div_split_key put zeros above every bound).
-template <int N>
+// dividends get the zero of the right sign.  A minimum over |q| guards the
+// group, and with kCheckU a maximum over |u| (fminf and fmaxf ignore NaN, and
+// a NaN dividend gives a NaN quotient on every path); the minimum starts at
+// |zh|, which is at least 2^-61 for a divisor with a split and 0 for one
+// without.  The guard also fires on zero quotients, the common case once a
+// wire has settled; a second test inside it looks for what the split does not
+// cover, and then the whole group goes to div_fixed: a divisor without a
+// split, an infinite u (infinite dividends, quotients beyond about 2^150), or
+// a nonzero dividend below |b| 2^-98 (a quotient that may be below
+// kDivSplitMin; the integer keys of div_split_key put zeros above every
+// bound).  Without kCheckU the caller guarantees finite u (see split_step).
+template <int N, bool kCheckU>
 __device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[N],
                                                 const SplitDivisor& d) {
   float m = fabsf(d.zh), M = 0.f;
@@ -103,14 +113,14 @@ __device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[
     float u;
     q[i] = div_split_q(a[i], d.zh, d.zl, &u);
     m = fminf(m, fabsf(q[i]));
-    M = fmaxf(M, fabsf(u));
+    if (kCheckU) M = fmaxf(M, fabsf(u));
   }
-  if (__builtin_expect(m < kDivSplitMin || M > __FLT_MAX__, 0)) {
+  if (__builtin_expect(m < kDivSplitMin || (kCheckU && M > __FLT_MAX__), 0)) {
     asm volatile("");
     uint32_t lo = 0xffffffffu;
 #pragma unroll
     for (int i = 0; i < N; i++) lo = min(lo, div_split_key(a[i]));
-    if (lo < div_split_key_bound(d.b) || M > __FLT_MAX__ || d.zh == 0.f) {
+    if (lo < div_split_key_bound(d.b) || (kCheckU && M > __FLT_MAX__) || d.zh == 0.f) {
       const double rb = 1.0 / (double)d.b;
 #pragma unroll
       for (int i = 0; i < N; i++) q[i] = div_fixed(a[i], rb, d.b);
@@ -118,12 +128,52 @@ __device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[
   }
 }
 
+// One step of the reference (circuit_gpu.cu:114-129) with div_split_group.
+// kCheckU = false needs every current and internal voltage entering the step,
+// L, I0, V0 and the node voltages at most kStepBound in magnitude (or NaN) and
+// R, C in [2^-40, 2^60] (make_div_split refuses others; their groups go to
+// div_fixed): then |L (I - I0)| <= 2^81, its quotient by dt <= 2^101, the R
+// dividends <= 2^102, their quotients <= 2^142, the C dividends <= 2^124, so
+// no dividend is infinite and every u = RN(-a zl) is finite (|zl| <= 2^-24
+// |zh| <= 2^16); the C quotients may overflow, which the fma rounds as IEEE
+// division does.
+constexpr float kStepBound = 0x1p40f;
+template <bool kCheckU>
+__device__ __forceinline__ void split_step(float (&ni)[kSeg], float (&nv)[kSeg + 1],
+                                           const float (&I0)[kSeg], const float (&V0)[kSeg - 1],
+                                           float L, const SplitDivisor& sdt,
+                                           const SplitDivisor& sR, const SplitDivisor& sC) {
+  const float dt = CKT_DT_F;
+  float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
+  div_split_group<kSeg, kCheckU>(q, t, sdt);
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
+  div_split_group<kSeg, kCheckU>(ni, t, sR);
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
+  div_split_group<kSeg - 1, kCheckU>(w, u, sC);
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
+}
+
+// Largest |value| of the running currents and internal voltages (NaN ignored).
+__device__ __forceinline__ float state_max(const float (&ni)[kSeg], const float (&nv)[kSeg + 1]) {
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) s = fmaxf(s, fabsf(ni[i]));
+#pragma unroll
+  for (int i = 1; i < kSeg; i++) s = fmaxf(s, fabsf(nv[i]));
+  return s;
+}
+
 // The reference's step in its order of operations (circuit_gpu.cu:114-129),
 // `steps` times, on the running currents ni and voltages nv (nv[0], nv[kSeg]
 // the node voltages), with I0 and V0 the wire's values at the start of the
 // loop; writes the result to I0 and V0.  Every division is correctly rounded:
 // the ten by dt, then the ten by R, then the nine by C, each group with one
-// guard; by the FP32 split (div_split_group, after a per-wire check of the
+// guard; by the FP32 split (split_step, after a per-wire check of the
 // divisors) when kSplit, else by the FP64 reciprocal (div_fixed_group, no
 // set-up: the short tails of the other policies).
 template <bool kSplit>
@@ -133,19 +183,25 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
   const float dt = CKT_DT_F;
   if constexpr (kSplit) {
     const SplitDivisor sdt(dt), sR(R), sC(C);
-    for (int j = 0; j < steps; j++) {
-      float t[kSeg], q[kSeg], u[kSeg - 1], w[kSeg - 1];
+    // The loop's constants against kStepBound, once; the running state after
+    // every step.  A wave whose lanes are all within it takes the step without
+    // the |u| guard.
+    float cmax = fmaxf(fabsf(L), fmaxf(fabsf(nv[0]), fabsf(nv[kSeg])));
 #pragma unroll
-      for (int i = 0; i < kSeg; i++) t[i] = L * (ni[i] - I0[i]);
-      div_split_group(q, t, sdt);
+    for (int i = 0; i < kSeg; i++) cmax = fmaxf(cmax, fabsf(I0[i]));
 #pragma unroll
-      for (int i = 0; i < kSeg; i++) t[i] = (nv[i + 1] - nv[i]) - q[i];
-      div_split_group(ni, t, sR);
-#pragma unroll
-      for (int i = 0; i < kSeg - 1; i++) u[i] = dt * (ni[i] - ni[i + 1]);
-      div_split_group(w, u, sC);
-#pragma unroll
-      for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
+    for (int i = 0; i < kSeg - 1; i++) cmax = fmaxf(cmax, fabsf(V0[i]));
+    const bool consts_ok = cmax <= kStepBound;
+    int j = 0;
+    while (j < steps) {
+      while (j < steps && wave_all(consts_ok && state_max(ni, nv) <= kStepBound)) {
+        split_step<false>(ni, nv, I0, V0, L, sdt, sR, sC);
+        j++;
+      }
+      while (j < steps && !wave_all(consts_ok && state_max(ni, nv) <= kStepBound)) {
+        split_step<true>(ni, nv, I0, V0, L, sdt, sR, sC);
+        j++;
+      }
     }
   } else {
     constexpr double rdt = 1.0 / (double)CKT_DT_F;
@@ -306,15 +362,6 @@ __device__ __forceinline__ bool tail_stays_finite(const float (&I)[kSeg], const 
   for (int i = 0; i < kSeg - 1; i++) ok &= fabsf(V[i]) <= 0x1p40f;
   return ok;
 }
-
-// Whether `p` holds on every active lane of the wave.  The policies skip the
-// tail only when the whole wave may: skipping changes the loop's trip count,
-// and a trip count that differs between lanes turns the counter of the
-// step loop into a per-lane register with an exec-mask update every
-// iteration (+0.5 VALU per step for SCALED, measured 2.6% more instructions
-// and 6% more cycles).  Running the tail on a wire that could skip it is
-// also correct.
-__device__ __forceinline__ bool wave_all(bool p) { return __all(p); }
 
 // The internal voltages formed from the final currents x by the reference's
 // voltage update (circuit.cc:398-401), V0_k + dt (x_{k-1} - x_k) / C, with
@@ -833,8 +880,8 @@ __global__ __launch_bounds__(1024) void dsc_lds_kernel(const DevCircuit* __restr
   // compiler cannot tell apart from the descriptor table, so fields read
   // through it would be reloaded after every atomic.
   const uint32_t ws = circuits[pt.circuit].ws;
-  const uint32_t num_wires = circuits[pt.circuit].num_wires;
-  const uint32_t num_nodes = circuits[pt.circuit].num_nodes;
+  [[maybe_unused]] const uint32_t num_wires = circuits[pt.circuit].num_wires;
+  [[maybe_unused]] const uint32_t num_nodes = circuits[pt.circuit].num_nodes;
   if (!CKT_IN_BOUNDS((uint64_t)p.first_node + p.num_nodes <= num_nodes)) return;
   const gfloat* cur0 = (const gfloat*)circuits[pt.circuit].cur;
   const gfloat* cur9 = cur0 + (size_t)(kSeg - 1) * ws;

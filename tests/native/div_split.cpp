@@ -67,7 +67,29 @@ static bool base_split_fails(uint32_t Yi) {
   return !div_split_candidates_pass(Yi, zh, zl0, 5, 10);
 }
 
-int main() {
+// With arguments: prints "<divisor> base|moved|refused" for each (strtof
+// syntax, hexadecimal floats included) and exits.
+static int classify(int argc, char** argv) {
+  for (int i = 1; i < argc; i++) {
+    const float y = std::strtof(argv[i], nullptr);
+    float zh, zl;
+    const char* kind = "refused";
+    if (make_div_split(y, &zh, &zl)) {
+      const uint32_t Yi = (f32_bits(fabsf(y)) & 0x7fffffu) | 0x800000u;
+      const float Y = significand_float(Yi);
+      const float zh0 = 1.0f / Y;
+      const float zl0 = (float)((1.0 - (double)Y * (double)zh0) / (double)Y);
+      const int e = ilogbf(fabsf(y));
+      const float base = Yi == 0x800000u ? zl : (y < 0.f ? -1.f : 1.f) * ldexpf(zl0, -e);
+      kind = f32_bits(zl) == f32_bits(base) ? "base" : "moved";
+    }
+    std::printf("%s %s\n", argv[i], kind);
+  }
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1) return classify(argc, argv);
   std::mt19937_64 rng(2004);
   // 1. Dividend significands, exhaustively, for the step's divisors and edges.
   check_divisor_exhaustive((float)1e-6, "dt");

@@ -22,11 +22,12 @@ division; this test checks the normal range, with random operands, with the
 divisors the circuit uses (dt, R in [1, 11], C in [0.001, 0.011]) and with
 dividends placed within two ulps of b*m for random midpoints m.
 """
+import json
 import os
 import subprocess
-import tempfile
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -81,18 +82,39 @@ def test_dividends_next_to_midpoints():
         check(a, b)
 
 
-def test_split_division_exhaustive():
+@pytest.fixture(scope="module")
+def div_split_exe(tmp_path_factory):
     from legionorigins_amd import build as B
-    with tempfile.TemporaryDirectory() as d:
-        exe = os.path.join(d, "div_split")
-        cmd = [B.hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "--offload-arch=gfx950", "-x", "hip",
-               "-I" + os.path.join(ROOT, "legionorigins_amd", "csrc"),
-               os.path.join(ROOT, "tests", "native", "div_split.cpp"), "-o", exe]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        assert r.returncode == 0, r.stderr[-3000:]
-        env = dict(os.environ)
-        env.pop("LD_PRELOAD", None)
-        r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
-        print(r.stdout)
-        assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
-        assert "split division: ok" in r.stdout
+    exe = str(tmp_path_factory.mktemp("div_split") / "div_split")
+    cmd = [B.hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "--offload-arch=gfx950", "-x", "hip",
+           "-I" + os.path.join(ROOT, "legionorigins_amd", "csrc"),
+           os.path.join(ROOT, "tests", "native", "div_split.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _run(exe, *args):
+    env = dict(os.environ)
+    env.pop("LD_PRELOAD", None)
+    return subprocess.run([exe, *args], capture_output=True, text=True, env=env, timeout=600)
+
+
+def test_split_division_exhaustive(div_split_exe):
+    r = _run(div_split_exe)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "split division: ok" in r.stdout
+
+
+def test_planted_divisors_are_classified_as_recorded(div_split_exe):
+    """The GPU test that plants these divisors in circuits relies on their
+    kinds: refused ones take the FP64 path, moved ones the moved split."""
+    fix = json.load(open(os.path.join(ROOT, "tests", "golden", "division_divisors.json")))
+    want = {"out_of_range": "refused", "power_of_two": "base"}
+    args = [(k, v) for k in ("base", "moved", "refused", "power_of_two", "out_of_range") for v in fix[k]]
+    r = _run(div_split_exe, *[v for _, v in args])
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = dict(line.split() for line in r.stdout.split("\n") if line.strip())
+    for k, v in args:
+        assert got[v] == want.get(k, k), (k, v, got[v])

@@ -1,0 +1,78 @@
+"""EXACT's division on the divisors its FP32 split does not take the common way
+(legionorigins_amd/csrc/ckt_divsplit.h): resistances and capacitances with no
+verified split (their quotients go through the FP64 path), outside the
+split's range [2^-40, 2^60], powers of two (exact reciprocal), and those that
+need the moved split, planted in C1-size circuits of both profiles so that
+most waves hold wires of several kinds.  Capacitance-sized values planted as
+resistances make some stable-profile wires grow until they overflow, which
+sends infinite dividends through these divisors too (the reference profile
+does that for every wire).  One calc_new_currents launch of a
+full loop (10 000 steps), compared with the oracle's calculate_currents task
+body (circuit.cc:352-413, GPU seeding) bit for bit on every current and
+internal voltage, non-finite classes included.  The divisors are in
+tests/golden/division_divisors.json; test_division_method.py checks their
+classification on the CPU.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import assert_bit_identical
+
+pytestmark = pytest.mark.gpu
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "division_divisors.json")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _planted(O, profile, seed, every_r, every_c, kinds):
+    circ = O.generate(4, 1024, 4096, 95, seed, profile)
+    vals = np.array([float.fromhex(v) for k in kinds for v in FIX[k]], np.float32)
+    w = circ.wires
+    idx_r = np.arange(0, len(w), every_r)
+    idx_c = np.arange(3, len(w), every_c)
+    w["resistance"][idx_r] = vals[np.arange(len(idx_r)) % len(vals)]
+    w["capacitance"][idx_c] = vals[(np.arange(len(idx_c)) * 7 + 3) % len(vals)]
+    return circ
+
+
+def _exact_gpu(C, circ):
+    with C.Context(0) as ctx:
+        ctx.load(C.HostCircuit(circ.nodes, circ.wires, circ.locator, circ.pieces))
+        ctx.calc_new_currents(ctx.pieces(), C.POLICY_EXACT)
+        return ctx.store()[1]
+
+
+def _oracle(O, circ):
+    ref = circ.copy()
+    for p in circ.pieces:
+        O.calc_new_currents(ref, p)
+    return ref.wires
+
+
+CASES = [
+    # profile, seed, every R, every C, divisor kinds
+    (1, 31, 5, 7, ("refused", "moved", "power_of_two", "base")),
+    (0, 32, 5, 7, ("refused", "moved", "power_of_two", "base")),
+    (1, 33, 3, 4, ("refused",)),
+    (1, 34, 11, 13, ("out_of_range", "refused")),
+    (1, 35, 1, 1, ("refused",)),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"prof{c[0]}-R{c[2]}-C{c[3]}-{'+'.join(c[4])}" for c in CASES])
+def test_exact_planted_divisors_bit_identical(C, O, case):
+    profile, seed, every_r, every_c, kinds = case
+    say(f"[divsplit] profile {profile}, seed {seed}, R every {every_r}, C every {every_c}, {kinds}")
+    circ = _planted(O, profile, seed, every_r, every_c, kinds)
+    got = _exact_gpu(C, circ)
+    want = _oracle(O, circ)
+    for f in ("current", "voltage"):
+        assert_bit_identical(got[f], want[f], f"{kinds} {f}")
