@@ -131,12 +131,13 @@ __device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[
 // One step of the reference (circuit_gpu.cu:114-129) with div_split_group.
 // kCheckU = false needs every current and internal voltage entering the step,
 // L, I0, V0 and the node voltages at most kStepBound in magnitude (or NaN) and
-// R, C in [2^-40, 2^60] (make_div_split refuses others; their groups go to
-// div_fixed): then |L (I - I0)| <= 2^81, its quotient by dt <= 2^101, the R
-// dividends <= 2^102, their quotients <= 2^142, the C dividends <= 2^124, so
-// no dividend is infinite and every u = RN(-a zl) is finite (|zl| <= 2^-24
-// |zh| <= 2^16); the C quotients may overflow, which the fma rounds as IEEE
-// division does.
+// |R|, |C| >= 2^-40 (a group whose divisor has no split goes to div_fixed,
+// whose quotients obey the same bounds): then |L (I - I0)| <= 2^81, its
+// quotient by dt <= 2^101, the R dividends <= 2^102, their quotients <= 2^142,
+// the C dividends <= 2^124, so no dividend is infinite and every
+// u = RN(-a zl) is finite (|zl| <= 2^-24 |zh| <= 2^16); the C quotients may
+// overflow, which the fma rounds as IEEE division does.  (A zero R would make
+// the R quotients infinite and the C dividends with them.)
 constexpr float kStepBound = 0x1p40f;
 template <bool kCheckU>
 __device__ __forceinline__ void split_step(float (&ni)[kSeg], float (&nv)[kSeg + 1],
@@ -183,15 +184,15 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
   const float dt = CKT_DT_F;
   if constexpr (kSplit) {
     const SplitDivisor sdt(dt), sR(R), sC(C);
-    // The loop's constants against kStepBound, once; the running state after
-    // every step.  A wave whose lanes are all within it takes the step without
-    // the |u| guard.
+    // The loop's constants against split_step's conditions, once; the running
+    // state after every step.  A wave whose lanes all meet them takes the
+    // step without the |u| guard.
     float cmax = fmaxf(fabsf(L), fmaxf(fabsf(nv[0]), fabsf(nv[kSeg])));
 #pragma unroll
     for (int i = 0; i < kSeg; i++) cmax = fmaxf(cmax, fabsf(I0[i]));
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) cmax = fmaxf(cmax, fabsf(V0[i]));
-    const bool consts_ok = cmax <= kStepBound;
+    const bool consts_ok = cmax <= kStepBound && fabsf(R) >= 0x1p-40f && fabsf(C) >= 0x1p-40f;
     int j = 0;
     while (j < steps) {
       while (j < steps && wave_all(consts_ok && state_max(ni, nv) <= kStepBound)) {

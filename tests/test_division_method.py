@@ -111,10 +111,15 @@ def test_planted_divisors_are_classified_as_recorded(div_split_exe):
     """The GPU test that plants these divisors in circuits relies on their
     kinds: refused ones take the FP64 path, moved ones the moved split."""
     fix = json.load(open(os.path.join(ROOT, "tests", "golden", "division_divisors.json")))
-    want = {"out_of_range": "refused", "power_of_two": "base"}
-    args = [(k, v) for k in ("base", "moved", "refused", "power_of_two", "out_of_range") for v in fix[k]]
-    r = _run(div_split_exe, *[v for _, v in args])
+    want = {"out_of_range": "refused", "power_of_two": "base", "zero": "refused"}
+    args = [(k, v) for k in ("base", "moved", "refused", "power_of_two", "out_of_range", "zero")
+            for v in fix[k]]
+    negs = [v[1:] for v in fix["negative"]]
+    r = _run(div_split_exe, *[v for _, v in args], *fix["negative"], *negs)
     assert r.returncode == 0, r.stderr[-2000:]
     got = dict(line.split() for line in r.stdout.split("\n") if line.strip())
     for k, v in args:
         assert got[v] == want.get(k, k), (k, v, got[v])
+    for v, m in zip(fix["negative"], negs):
+        assert got[v] == got[m], (v, got[v], got[m])
+    assert {got[v] for v in fix["negative"]} == {"base", "moved", "refused"}

@@ -1,8 +1,8 @@
 """EXACT's division on the divisors its FP32 split does not take the common way
 (legionorigins_amd/csrc/ckt_divsplit.h): resistances and capacitances with no
 verified split (their quotients go through the FP64 path), outside the
-split's range [2^-40, 2^60], powers of two (exact reciprocal), and those that
-need the moved split, planted in C1-size circuits of both profiles so that
+split's range [2^-40, 2^60], zero, negative, powers of two (exact
+reciprocal), and those that need the moved split, planted in C1-size circuits of both profiles so that
 most waves hold wires of several kinds.  Capacitance-sized values planted as
 resistances make some stable-profile wires grow until they overflow, which
 sends infinite dividends through these divisors too (the reference profile
@@ -43,17 +43,22 @@ def _planted(O, profile, seed, every_r, every_c, kinds):
     return circ
 
 
-def _exact_gpu(C, circ):
+def _exact_gpu(C, circ, steps=None):
     with C.Context(0) as ctx:
         ctx.load(C.HostCircuit(circ.nodes, circ.wires, circ.locator, circ.pieces))
+        if steps is not None:
+            ctx.set_option(C.OPT_STEPS, steps)
         ctx.calc_new_currents(ctx.pieces(), C.POLICY_EXACT)
         return ctx.store()[1]
 
 
-def _oracle(O, circ):
+def _oracle(O, circ, steps=None):
     ref = circ.copy()
     for p in circ.pieces:
-        O.calc_new_currents(ref, p)
+        if steps is None:
+            O.calc_new_currents(ref, p)
+        else:
+            O.calc_new_currents(ref, p, steps=steps)
     return ref.wires
 
 
@@ -64,6 +69,7 @@ CASES = [
     (1, 33, 3, 4, ("refused",)),
     (1, 34, 11, 13, ("out_of_range", "refused")),
     (1, 35, 1, 1, ("refused",)),
+    (1, 36, 9, 10, ("zero", "negative", "moved")),
 ]
 
 
@@ -76,3 +82,24 @@ def test_exact_planted_divisors_bit_identical(C, O, case):
     want = _oracle(O, circ)
     for f in ("current", "voltage"):
         assert_bit_identical(got[f], want[f], f"{kinds} {f}")
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_exact_zero_and_tiny_divisors_first_steps(C, O, steps):
+    """A zero resistance makes that wire's currents infinite within the step,
+    and the step's capacitive divisions then take infinite dividends; a few
+    steps later the reference has turned such a wire to NaN, so the classes
+    are compared right after the first steps."""
+    say(f"[divsplit] zero and tiny divisors, {steps} steps")
+    circ = _planted(O, 1, 37, 9, 10, ("zero", "out_of_range", "negative"))
+    # start from mid-loop-like state: nonzero currents and internal voltages of
+    # both signs, so that a zero resistance gives infinite currents of both
+    # signs and the capacitive divisions infinite dividends
+    rng = np.random.default_rng(37)
+    circ.wires["voltage"] = rng.uniform(-1, 1, circ.wires["voltage"].shape).astype(np.float32)
+    circ.wires["current"] = rng.uniform(-0.1, 0.1, circ.wires["current"].shape).astype(np.float32)
+    got = _exact_gpu(C, circ, steps)
+    want = _oracle(O, circ, steps)
+    assert np.isinf(want["voltage"]).any()
+    for f in ("current", "voltage"):
+        assert_bit_identical(got[f], want[f], f"{steps} steps {f}")
