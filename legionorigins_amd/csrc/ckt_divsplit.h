@@ -1,6 +1,6 @@
 // ckt_divsplit.h -- correctly rounded FP32 division by a divisor that stays
-// fixed over many dividends, with FP32 operations only: one multiply and one
-// fma per quotient,
+// fixed over many dividends, with FP32 operations only: two fmas per quotient
+// (the first one a multiply),
 //
 //     q = RN(x * zh + RN(x * zl)),     zh + zl ~ 1/y,
 //
@@ -28,10 +28,10 @@
 // Xi = +-d 2^-(24-s) (mod Y').  For each odd d up to the bound these are one
 // or two residues, and every Xi of the case's range with one of them is a
 // candidate; if the split gives RN(X / Y) for every candidate, it gives it for
-// every X.  zh = RN(1/Y) and zl = RN(1/Y - zh) give E <= 1.5 2^-49; most
-// divisors pass with it, 99.8% with zl moved one ulp either way, which is also
-// tried.  The rest are refused (make_div_split
-// returns false) and their quotients are computed another way.
+// every X.  zh = RN(1/Y) and zl = RN(1/Y - zh) give E <= 1.5 2^-49; 98.7% of
+// divisors pass with it, 99.9% with zl moved one ulp either way, which is
+// also tried.  The rest are refused (make_div_split returns false) and their
+// quotients are computed another way.
 //
 // When the result holds for the actual operands.  The split of y is the split
 // of Y scaled by 2^-e (exact: |y| in [2^-40, 2^60] keeps zh and zl normal,
