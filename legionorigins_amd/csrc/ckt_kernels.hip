@@ -159,6 +159,31 @@ __device__ __forceinline__ void split_step(float (&ni)[kSeg], float (&nv)[kSeg +
   for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
 }
 
+// How many steps split_step<false> may run after one state check, for one
+// wire.  From a state and loop constants of magnitude at most S (NaN aside),
+// one step (circuit_gpu.cu:114-129, 1/dt < 2^20) gives
+//   |L (I - I0)| / dt <= 2^21 |L| S,  R dividends <= (2 + 2^21 |L|) S,
+//   new currents <= g2 S with g2 = (2 + 2^21 |L|) / |R|,
+//   C dividends <= 2 dt g2 S <= 2^-18 g2 S,  their quotients <= g3 S with
+//   g3 = 2^-18 g2 / |C|,  new internal voltages <= (1 + g3) S,
+// so the state grows by at most G = max(g2, 1 + g3) per step and every value
+// of a step stays within H S, H = max(2 + 2^21 |L|, g2, 1 + g3); every u is at
+// most 2^16 times its dividend (|zl| <= 2^-24 |zh| <= 2^16 for |R|, |C| >=
+// 2^-40).  From S <= kStepBound = 2^40, K steps keep every u below
+// 2^(40 + 16 + lg H + (K - 1) lg G), finite while lg H + (K - 1) lg G < 71
+// (lg v = ilogb(v) + 1 >= log2 v, rounding included).  1 for constants that
+// allow no more (or are not finite), at most 8.
+__device__ __forceinline__ int unchecked_steps(float L, float R, float C) {
+  const float a = fmaf(fabsf(L), 0x1p21f, 2.f);
+  const float g2 = a / fabsf(R);
+  const float g3 = g2 * 0x1p-18f / fabsf(C);
+  const float G = fmaxf(g2, 1.f + g3), H = fmaxf(fmaxf(a, g2), 1.f + g3);
+  if (!(G <= 0x1p60f && H <= 0x1p60f)) return 1;
+  const int lgG = ilogbf(fmaxf(G, 1.f)) + 1, lgH = ilogbf(fmaxf(H, 1.f)) + 1;
+  const int k = 1 + (70 - lgH) / lgG;
+  return k < 1 ? 1 : (k > 8 ? 8 : k);
+}
+
 // Largest |value| of the running currents and internal voltages (NaN ignored).
 __device__ __forceinline__ float state_max(const float (&ni)[kSeg], const float (&nv)[kSeg + 1]) {
   float s = 0.f;
@@ -185,19 +210,25 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
   if constexpr (kSplit) {
     const SplitDivisor sdt(dt), sR(R), sC(C);
     // The loop's constants against split_step's conditions, once; the running
-    // state after every step.  A wave whose lanes all meet them takes the
-    // step without the |u| guard.
+    // state every K steps.  A wave whose lanes all meet them takes K steps
+    // without the |u| guard.
     float cmax = fmaxf(fabsf(L), fmaxf(fabsf(nv[0]), fabsf(nv[kSeg])));
 #pragma unroll
     for (int i = 0; i < kSeg; i++) cmax = fmaxf(cmax, fabsf(I0[i]));
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) cmax = fmaxf(cmax, fabsf(V0[i]));
     const bool consts_ok = cmax <= kStepBound && fabsf(R) >= 0x1p-40f && fabsf(C) >= 0x1p-40f;
+    // Steps per state check: the largest of 8, 4, 3, 2 that every active lane
+    // allows (votes, so inactive lanes take no part).
+    const int kl = unchecked_steps(L, R, C);
+    const int K = wave_all(kl >= 8) ? 8 : wave_all(kl >= 4) ? 4 : wave_all(kl >= 3) ? 3
+                : wave_all(kl >= 2) ? 2 : 1;
     int j = 0;
     while (j < steps) {
       while (j < steps && wave_all(consts_ok && state_max(ni, nv) <= kStepBound)) {
-        split_step<false>(ni, nv, I0, V0, L, sdt, sR, sC);
-        j++;
+        const int n = min(K, steps - j);
+        for (int k = 0; k < n; k++) split_step<false>(ni, nv, I0, V0, L, sdt, sR, sC);
+        j += n;
       }
       while (j < steps && !wave_all(consts_ok && state_max(ni, nv) <= kStepBound)) {
         split_step<true>(ni, nv, I0, V0, L, sdt, sR, sC);
