@@ -165,4 +165,37 @@ __host__ __device__ inline bool make_div_split(float y, float* zh_out, float* zl
   return false;
 }
 
+constexpr float kStepBound = 0x1p40f;
+
+// The EXACT step (circuit_gpu.cu:114-129) without its guard over |u| =
+// |RN(a zl)|: a wave may take it while every lane's running currents and
+// internal voltages and loop constants are within kStepBound, its |R| is at
+// least 2^-20 and its |C| at least 2^-40 (then no dividend or u is infinite;
+// see split_step), and for this many steps after one check of that state, for
+// one wire.  From a state and loop constants of magnitude at most S (NaN aside),
+// one step (circuit_gpu.cu:114-129, 1/dt < 2^20) gives
+//   |L (I - I0)| / dt <= 2^21 |L| S,  R dividends <= (2 + 2^21 |L|) S,
+//   new currents <= g2 S with g2 = (2 + 2^21 |L|) / |R|,
+//   C dividends <= 2 dt g2 S <= 2^-18 g2 S,  their quotients <= g3 S with
+//   g3 = 2^-18 g2 / |C|,  new internal voltages <= (1 + g3) S,
+// so the state grows by at most G = max(g2, 1 + g3) per step and every value
+// of a step stays within H S, H = max(2 + 2^21 |L|, g2, 1 + g3); every u is at
+// most 2^16 times its dividend (|zl| <= 2^-24 |zh| <= 2^16 for |R|, |C| >=
+// 2^-40).  From S <= kStepBound = 2^40, K steps keep every value below
+// 2^(40 + lg H + (K - 1) lg G) and every u below 2^16 times that, finite
+// while lg H + (K - 1) lg G < 71 (lg v = ilogb(v) + 1 >= log2 v, rounding
+// included); the last step's C quotients may overflow (the next step is
+// checked again).  1 for constants that allow no more (or are not finite),
+// at most 8; one step is always allowed under the conditions above.
+__host__ __device__ inline int unchecked_steps(float L, float R, float C) {
+  const float a = fmaf(fabsf(L), 0x1p21f, 2.f);
+  const float g2 = a / fabsf(R);
+  const float g3 = g2 * 0x1p-18f / fabsf(C);
+  const float G = fmaxf(g2, 1.f + g3), H = fmaxf(fmaxf(a, g2), 1.f + g3);
+  if (!(G <= 0x1p60f && H <= 0x1p60f)) return 1;
+  const int lgG = ilogbf(fmaxf(G, 1.f)) + 1, lgH = ilogbf(fmaxf(H, 1.f)) + 1;
+  const int k = 1 + (70 - lgH) / lgG;
+  return k < 1 ? 1 : (k > 8 ? 8 : k);
+}
+
 }  // namespace ckt

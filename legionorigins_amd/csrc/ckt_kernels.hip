@@ -130,15 +130,15 @@ __device__ __forceinline__ void div_split_group(float (&q)[N], const float (&a)[
 
 // One step of the reference (circuit_gpu.cu:114-129) with div_split_group.
 // kCheckU = false needs every current and internal voltage entering the step,
-// L, I0, V0 and the node voltages at most kStepBound in magnitude (or NaN) and
-// |R|, |C| >= 2^-40 (a group whose divisor has no split goes to div_fixed,
-// whose quotients obey the same bounds): then |L (I - I0)| <= 2^81, its
-// quotient by dt <= 2^101, the R dividends <= 2^102, their quotients <= 2^142,
-// the C dividends <= 2^124, so no dividend is infinite and every
-// u = RN(-a zl) is finite (|zl| <= 2^-24 |zh| <= 2^16); the C quotients may
-// overflow, which the fma rounds as IEEE division does.  (A zero R would make
-// the R quotients infinite and the C dividends with them.)
-constexpr float kStepBound = 0x1p40f;
+// L, I0, V0 and the node voltages at most kStepBound in magnitude (or NaN),
+// |R| >= 2^-20 and |C| >= 2^-40 (a group whose divisor has no split goes to
+// div_fixed, whose quotients obey the same bounds): then |L (I - I0)| <=
+// 2^81, its quotient by dt <= 2^101, the R dividends <= 2^102, their
+// quotients <= 2^122, the C dividends <= 2^104, so no dividend is infinite and
+// every u = RN(-a zl) is finite (|zl| <= 2^-24 |zh|: <= 2^-4 for dt, 2^-4 for
+// R, 2^16 for C); the C quotients may overflow, which the fma rounds as IEEE
+// division does.  (A smaller R lets the R quotients overflow, and the C
+// dividends become infinite with them.)
 template <bool kCheckU>
 __device__ __forceinline__ void split_step(float (&ni)[kSeg], float (&nv)[kSeg + 1],
                                            const float (&I0)[kSeg], const float (&V0)[kSeg - 1],
@@ -157,31 +157,6 @@ __device__ __forceinline__ void split_step(float (&ni)[kSeg], float (&nv)[kSeg +
   div_split_group<kSeg - 1, kCheckU>(w, u, sC);
 #pragma unroll
   for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V0[i] + w[i];
-}
-
-// How many steps split_step<false> may run after one state check, for one
-// wire.  From a state and loop constants of magnitude at most S (NaN aside),
-// one step (circuit_gpu.cu:114-129, 1/dt < 2^20) gives
-//   |L (I - I0)| / dt <= 2^21 |L| S,  R dividends <= (2 + 2^21 |L|) S,
-//   new currents <= g2 S with g2 = (2 + 2^21 |L|) / |R|,
-//   C dividends <= 2 dt g2 S <= 2^-18 g2 S,  their quotients <= g3 S with
-//   g3 = 2^-18 g2 / |C|,  new internal voltages <= (1 + g3) S,
-// so the state grows by at most G = max(g2, 1 + g3) per step and every value
-// of a step stays within H S, H = max(2 + 2^21 |L|, g2, 1 + g3); every u is at
-// most 2^16 times its dividend (|zl| <= 2^-24 |zh| <= 2^16 for |R|, |C| >=
-// 2^-40).  From S <= kStepBound = 2^40, K steps keep every u below
-// 2^(40 + 16 + lg H + (K - 1) lg G), finite while lg H + (K - 1) lg G < 71
-// (lg v = ilogb(v) + 1 >= log2 v, rounding included).  1 for constants that
-// allow no more (or are not finite), at most 8.
-__device__ __forceinline__ int unchecked_steps(float L, float R, float C) {
-  const float a = fmaf(fabsf(L), 0x1p21f, 2.f);
-  const float g2 = a / fabsf(R);
-  const float g3 = g2 * 0x1p-18f / fabsf(C);
-  const float G = fmaxf(g2, 1.f + g3), H = fmaxf(fmaxf(a, g2), 1.f + g3);
-  if (!(G <= 0x1p60f && H <= 0x1p60f)) return 1;
-  const int lgG = ilogbf(fmaxf(G, 1.f)) + 1, lgH = ilogbf(fmaxf(H, 1.f)) + 1;
-  const int k = 1 + (70 - lgH) / lgG;
-  return k < 1 ? 1 : (k > 8 ? 8 : k);
 }
 
 // Largest |value| of the running currents and internal voltages (NaN ignored).
@@ -217,7 +192,7 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
     for (int i = 0; i < kSeg; i++) cmax = fmaxf(cmax, fabsf(I0[i]));
 #pragma unroll
     for (int i = 0; i < kSeg - 1; i++) cmax = fmaxf(cmax, fabsf(V0[i]));
-    const bool consts_ok = cmax <= kStepBound && fabsf(R) >= 0x1p-40f && fabsf(C) >= 0x1p-40f;
+    const bool consts_ok = cmax <= kStepBound && fabsf(R) >= 0x1p-20f && fabsf(C) >= 0x1p-40f;
     // Steps per state check: the largest of 8, 4, 3, 2 that every active lane
     // allows (votes, so inactive lanes take no part).
     const int kl = unchecked_steps(L, R, C);

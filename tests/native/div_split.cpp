@@ -1,5 +1,6 @@
 // Host checks of the split division (ckt_divsplit.h), the division of the
-// EXACT calc_new_currents step: for every divisor make_div_split accepts,
+// EXACT calc_new_currents step, and of the bound on its unguarded steps
+// (unchecked_steps): for every divisor make_div_split accepts,
 // div_split_q(x) == x / y (IEEE, correctly rounded) for every dividend
 // significand, for random finite operands outside the kernel's exact-path
 // tests, and for zero dividends of either sign.  Built and run by tests/test_division_method.py;
@@ -86,6 +87,71 @@ static int classify(int argc, char** argv) {
     std::printf("%s %s\n", argv[i], kind);
   }
   return 0;
+}
+
+// The EXACT step's unguarded runs: from a state and loop constants within
+// kStepBound, unchecked_steps(L, R, C) steps of the reference recurrence
+// (circuit_gpu.cu:114-129, IEEE arithmetic) give no infinite dividend,
+// quotient or u = RN(-a zl) (for a divisor with a split), except that the
+// last step's capacitive quotients may overflow.  States near the
+// bound with random signs; constants over the whole allowed range and the
+// generator's.  Returns a histogram of K.
+static void check_unchecked_steps(std::mt19937_64& rng, long hist[9]) {
+  const float dt = (float)1e-6;
+  std::uniform_real_distribution<double> u01(0, 1);
+  auto logu = [&](double lo, double hi) { return (float)std::exp2(lo + (hi - lo) * u01(rng)); };
+  auto sgn = [&]() { return (rng() & 1) ? 1.f : -1.f; };
+  float zhT, zlT;
+  make_div_split(dt, &zhT, &zlT);
+  for (int it = 0; it < 300000; it++) {
+    float L, R, C;
+    if (it % 3 == 0) {
+      L = logu(-30, 40); R = logu(-40, 60); C = logu(-40, 60);
+    } else if (it % 3 == 1) {
+      L = (float)(u01(rng) * 0.01 + 0.1); R = (float)(u01(rng) * 10 + 1); C = (float)(u01(rng) * 0.1);
+    } else {
+      L = (float)((u01(rng) * 0.01 + 0.1) * 1e-6); R = (float)(u01(rng) * 10 + 1);
+      C = (float)(u01(rng) * 0.01 + 0.001);
+    }
+    L *= sgn(); R *= sgn(); C *= sgn();
+    if (!(fabsf(R) >= 0x1p-20f && fabsf(C) >= 0x1p-40f)) continue;  // the kernel's conditions
+    const int K = unchecked_steps(L, R, C);
+    CHECK(K >= 1 && K <= 8, "K = %d", K);
+    hist[K]++;
+    if (it % 3 == 1) CHECK(K >= 3 || fabsf(C) < 1e-5f, "reference-profile constants give K = %d", K);
+    if (it % 3 == 2) CHECK(K == 8, "stable-profile constants give K = %d", K);
+    float zhR, zlR, zhC, zlC;
+    const bool okR = make_div_split(R, &zhR, &zlR), okC = make_div_split(C, &zhC, &zlC);
+    const float S = it % 7 ? kStepBound : logu(0, 40);
+    float I0[10], V0[9], ni[10], nv[11];
+    for (int i = 0; i < 10; i++) { ni[i] = S * sgn(); I0[i] = S * sgn(); }
+    for (int i = 0; i < 9; i++) V0[i] = S * sgn();
+    for (int i = 0; i < 11; i++) nv[i] = (i == 0 || i == 10 || it % 2) ? S * sgn() : V0[i - 1];
+    bool bad = false;
+    for (int k = 0; k < K && !bad; k++) {
+      float q[10], w[9];
+      for (int i = 0; i < 10; i++) {
+        const float t = L * (ni[i] - I0[i]);
+        q[i] = t / dt;
+        bad |= !std::isfinite(t) || !std::isfinite(q[i]) || !std::isfinite(fmaf(-t, zlT, 0.f));
+      }
+      for (int i = 0; i < 10; i++) {
+        const float a = (nv[i + 1] - nv[i]) - q[i];
+        ni[i] = a / R;
+        bad |= !std::isfinite(a) || !std::isfinite(ni[i]) || (okR && !std::isfinite(fmaf(-a, zlR, 0.f)));
+      }
+      for (int i = 0; i < 9; i++) {
+        const float a = dt * (ni[i] - ni[i + 1]);
+        w[i] = a / C;
+        // the last step's capacitive quotients may overflow (the fma rounds
+        // them as IEEE division does); the next step is checked again
+        bad |= !std::isfinite(a) || (k + 1 < K && !std::isfinite(w[i])) ||
+               (okC && !std::isfinite(fmaf(-a, zlC, 0.f)));
+      }
+      for (int i = 0; i < 9; i++) nv[i + 1] = V0[i] + w[i];
+    }
+    CHECK(!bad, "L=%a R=%a C=%a S=%a: a value left the finite range within K=%d steps", L, R, C, S, K);
+  }
 }
 
 int main(int argc, char** argv) {
@@ -177,6 +243,11 @@ int main(int argc, char** argv) {
               (double)acc / N, (double)accR / N, (double)accC / N, (double)accC2 / N);
   CHECK(acc > 0.998 * N && accR > 0.998 * N && accC > 0.998 * N && accC2 > 0.998 * N,
         "fewer than 99.8%% of divisors get a split");
+  long hist[9] = {0};
+  check_unchecked_steps(rng, hist);
+  std::printf("  unguarded steps per state check:");
+  for (int k = 1; k <= 8; k++) std::printf(" K=%d: %ld", k, hist[k]);
+  std::printf("\n");
   if (fails) return 1;
   std::printf("split division: ok\n");
   return 0;

@@ -103,3 +103,34 @@ def test_exact_zero_and_tiny_divisors_first_steps(C, O, steps):
     assert np.isinf(want["voltage"]).any()
     for f in ("current", "voltage"):
         assert_bit_identical(got[f], want[f], f"{steps} steps {f}")
+
+
+def _overflowing_circuit(O):
+    circ = O.generate(4, 1024, 4096, 95, 38, 1)
+    w = circ.wires
+    rng = np.random.default_rng(38)
+    idx = np.arange(0, len(w), 9)
+    n = len(idx)
+    w["resistance"][idx] = (rng.uniform(1, 2, n) * 2.0 ** -38 * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+    w["inductance"][idx] = np.float32(2.0 ** 39)
+    w["current"][idx] = 0
+    w["voltage"][idx] = rng.uniform(-0.5, 0.5, (n, 9)).astype(np.float32)
+    return circ, idx
+
+
+def test_exact_overflowing_resistive_quotients(C, O):
+    """Resistances near 2^-38 with L = 2^39 (every 9th wire): the first step
+    leaves currents up to ~2^39, within the bound the step without the |u|
+    guard starts from, and in the second the inductive term reaches ~2^98,
+    the resistive quotients overflow and the capacitive divisions take
+    infinite dividends (that step is not allowed below |R| = 2^-20)."""
+    say("[divsplit] overflowing resistive quotients, 2 steps")
+    circ, idx = _overflowing_circuit(O)
+    after1 = _oracle(O, circ, 1)
+    assert np.abs(after1["current"][idx]).max() <= 2.0 ** 40
+    assert np.abs(after1["voltage"][idx]).max() <= 2.0 ** 40
+    got = _exact_gpu(C, circ, 2)
+    want = _oracle(O, circ, 2)
+    assert np.isinf(want["current"][idx]).any() and np.isinf(want["voltage"][idx]).any()
+    for f in ("current", "voltage"):
+        assert_bit_identical(got[f], want[f], f"2 steps {f}")
