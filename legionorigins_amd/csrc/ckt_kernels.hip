@@ -234,19 +234,27 @@ __device__ __forceinline__ void reference_steps(float (&ni)[kSeg], float (&nv)[k
   for (int i = 0; i < kSeg - 1; i++) V0[i] = nv[i + 1];
 }
 
+// A whole loop in the reference's order, seeded as the reference kernel
+// seeds it (circuit_gpu.cu:107-112); kSplit as for reference_steps.
+template <bool kSplit>
+__device__ __forceinline__ void reference_loop(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
+                                               float vout, float L, float R, float C, int steps) {
+  float ni[kSeg], nv[kSeg + 1];
+#pragma unroll
+  for (int i = 0; i < kSeg; i++) ni[i] = I[i];
+  nv[0] = vin;
+#pragma unroll
+  for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
+  nv[kSeg] = vout;
+  reference_steps<kSplit>(ni, nv, I, V, L, R, C, steps);
+}
+
 // Reference order of operations; bit-identical to a strict FP32 CPU run.
 template <>
 struct Relax<CKT_POLICY_EXACT> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
-    float ni[kSeg], nv[kSeg + 1];
-#pragma unroll
-    for (int i = 0; i < kSeg; i++) ni[i] = I[i];
-    nv[0] = vin;
-#pragma unroll
-    for (int i = 0; i < kSeg - 1; i++) nv[i + 1] = V[i];
-    nv[kSeg] = vout;
-    reference_steps<true>(ni, nv, I, V, L, R, C, steps);
+    reference_loop<true>(I, V, vin, vout, L, R, C, steps);
   }
 };
 
@@ -418,7 +426,7 @@ struct Relax<CKT_POLICY_RECIP> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
     if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
-      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
+      reference_loop<false>(I, V, vin, vout, L, R, C, steps);
       return;
     }
     const bool no_tail = wave_all(tail_stays_finite(I, V, vin, vout, L, R, C));
@@ -476,7 +484,7 @@ struct Relax<CKT_POLICY_FAST> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
     if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
-      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
+      reference_loop<false>(I, V, vin, vout, L, R, C, steps);
       return;
     }
     const bool no_tail = wave_all(tail_stays_finite(I, V, vin, vout, L, R, C));
@@ -629,7 +637,7 @@ struct Relax<CKT_POLICY_SCALED> {
   static __device__ __forceinline__ void run(float (&I)[kSeg], float (&V)[kSeg - 1], float vin,
                                              float vout, float L, float R, float C, int steps) {
     if (whole_loop_in_reference_order(I, V, vin, vout, L, R, C, steps)) {
-      Relax<CKT_POLICY_EXACT>::run(I, V, vin, vout, L, R, C, steps);
+      reference_loop<false>(I, V, vin, vout, L, R, C, steps);
       return;
     }
     const bool no_tail = wave_all(tail_stays_finite(I, V, vin, vout, L, R, C));
