@@ -740,8 +740,11 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
   // Launch shape unless set with OPT_PERSISTENT / OPT_WAVES_PER_SIMD.  On the
   // 64 x 16384 x 65536 circuit a persistent tile queue with 4 resident waves
   // per SIMD beats the plain grid by 6% for SCALED and 1-2% for FAST and
-  // RECIP; EXACT, whose FP64 multiplies need every resident wave to hide
-  // their latency, keeps the plain grid (6% slower at 4 waves).
+  // RECIP.  EXACT keeps the plain grid of 256-lane workgroups: a persistent
+  // queue of them is 1-4% slower (10% at 4 waves); one of 64-lane workgroups
+  // at 7 waves per SIMD is 1-3% faster (OPT_CNC_BLOCK 64, OPT_PERSISTENT 1,
+  // OPT_WAVES_PER_SIMD 7), but the tile lists a loop prefetches before graph
+  // capture are built for one workgroup size for every policy.
   const bool persistent = persistent_ >= 0 ? persistent_ != 0 : policy != CKT_POLICY_EXACT;
   const int waves = waves_per_simd_ > 0 ? waves_per_simd_ : (policy == CKT_POLICY_EXACT ? 8 : 4);
   if (persistent) {
