@@ -331,11 +331,13 @@ int ckt_run_loops(ckt_ctx_t* ctx, int num_loops, const ckt_loop_config_t* config
 /* Tuning and test options (context and batch). */
 enum {
   CKT_OPT_STEPS = 0,             /* inner steps per loop (default CKT_STEPS) */
-  CKT_OPT_CNC_BLOCK = 1,         /* calc_new_currents workgroup size: 64/128/256/512 */
+  CKT_OPT_CNC_BLOCK = 1,         /* calc_new_currents workgroup size: 64/128/256/512;
+                                    -1 (default): 64 for EXACT, 256 otherwise */
   CKT_OPT_PERSISTENT = 2,        /* 1: persistent tile-queue calc_new_currents, 0: plain
-                                    grid, -1 (default): persistent except for EXACT */
+                                    grid, -1 (default): persistent, except for EXACT with
+                                    its workgroup size set */
   CKT_OPT_WAVES_PER_SIMD = 3,    /* persistent grid: waves resident per SIMD (1..8);
-                                    -1 (default): 8 for EXACT, 4 otherwise */
+                                    -1 (default): 7 for EXACT, 4 otherwise */
   CKT_OPT_DSC_ATOMIC_IMPL = 4,   /* ATOMIC reduction: 0 auto, 1 global atomics, 2 LDS */
   CKT_OPT_FUSE_DSC = 5,          /* 1 (default): in ckt_run_loops (sync 0, per_piece 0),
                                     batches and servers, the ATOMIC distribute_charge is

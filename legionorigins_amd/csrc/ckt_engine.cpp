@@ -733,28 +733,30 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
   int st;
   if ((st = ensure_table())) return st;
   DevBuf t;
-  if ((st = tiles_for(TK_CNC, refs, cnc_block_, &t))) return st;
+  const int block = cnc_block_for(policy);
+  if ((st = tiles_for(TK_CNC, refs, block, &t))) return st;
   hipEvent_t a;
   if ((st = timed_begin(CKT_CALC_NEW_CURRENTS, &a))) return st;
   hipError_t e;
-  // Launch shape unless set with OPT_PERSISTENT / OPT_WAVES_PER_SIMD.  On the
-  // 64 x 16384 x 65536 circuit a persistent tile queue with 4 resident waves
-  // per SIMD beats the plain grid by 6% for SCALED and 1-2% for FAST and
-  // RECIP.  EXACT keeps the plain grid of 256-lane workgroups: a persistent
-  // queue of them is 1-4% slower (10% at 4 waves); one of 64-lane workgroups
-  // at 7 waves per SIMD is 1-3% faster (OPT_CNC_BLOCK 64, OPT_PERSISTENT 1,
-  // OPT_WAVES_PER_SIMD 7), but the tile lists a loop prefetches before graph
-  // capture are built for one workgroup size for every policy.
-  const bool persistent = persistent_ >= 0 ? persistent_ != 0 : policy != CKT_POLICY_EXACT;
-  const int waves = waves_per_simd_ > 0 ? waves_per_simd_ : (policy == CKT_POLICY_EXACT ? 8 : 4);
+  // Launch shape unless set with OPT_CNC_BLOCK / OPT_PERSISTENT /
+  // OPT_WAVES_PER_SIMD.  On the 64 x 16384 x 65536 circuit a persistent tile
+  // queue of 256-lane workgroups with 4 resident waves per SIMD beats the
+  // plain grid by 6% for SCALED and 1-2% for FAST and RECIP.  For EXACT (71
+  // VGPRs, at most 7 resident waves) a queue of 64-lane workgroups at 7 waves
+  // per SIMD is 1-3% faster than the plain grid of 256-lane workgroups, which
+  // it keeps when the workgroup size is set (a queue of 256-lane workgroups
+  // is 1-4% slower than that grid).
+  const bool persistent = persistent_ >= 0 ? persistent_ != 0
+                                           : policy != CKT_POLICY_EXACT || cnc_block_ < 0;
+  const int waves = waves_per_simd_ > 0 ? waves_per_simd_ : (policy == CKT_POLICY_EXACT ? 7 : 4);
   if (persistent) {
-    const int waves_per_block = cnc_block_ / 64;
+    const int waves_per_block = block / 64;
     int grid = num_cus_ * 4 * waves / waves_per_block;
     if (grid > (int)t.n) grid = (int)t.n;
-    e = launch_cnc_persistent(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, cnc_block_, grid,
+    e = launch_cnc_persistent(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, block, grid,
                               d_counters_, steps_, stream_);
   } else {
-    e = launch_cnc(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, cnc_block_, steps_, stream_);
+    e = launch_cnc(policy, fuse_dsc, d_table_, (const Tile*)t.p, t.n, block, steps_, stream_);
   }
   if (e != hipSuccess) return hip_fail(e, "calc_new_currents launch");
   return timed_end(CKT_CALC_NEW_CURRENTS, a);
@@ -763,7 +765,7 @@ int Engine::cnc(const std::vector<PieceRef>& refs, int policy, bool fuse_dsc) {
 // Every tile list one loop's launches use, built before the first launch.
 // A synchronous upload issued while a kernel runs waits for CUs the kernel
 // holds (for the whole of a calc_new_currents launch), and the host with it.
-int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
+int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int policy, int mode) {
   int st;
   if (mode < CKT_REDUCE_ATOMIC || mode > CKT_REDUCE_CSR_KAHAN64)
     return fail(CKT_ERR_INVALID_ARG, "distribute_charge: unknown reduction mode");
@@ -771,7 +773,7 @@ int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
   if (tile_cache_.size() + 8 > 64) drop_caches();
   if ((st = ensure_table())) return st;
   DevBuf t;
-  if ((st = tiles_for(TK_CNC, refs, cnc_block_, &t))) return st;
+  if ((st = tiles_for(TK_CNC, refs, cnc_block_for(policy), &t))) return st;
   if (mode == CKT_REDUCE_ATOMIC && !fuse_dsc_) {
     if ((st = tiles_for(TK_WIRES, refs, 256, &t))) return st;
     LdsPlan plan;
@@ -787,7 +789,7 @@ int Engine::prefetch_tiles(const std::vector<PieceRef>& refs, int mode) {
 
 int Engine::loop_once(const std::vector<PieceRef>& refs, int policy, int mode) {
   int st;
-  if ((st = prefetch_tiles(refs, mode))) return st;
+  if ((st = prefetch_tiles(refs, policy, mode))) return st;
   return launch_loop(refs, policy, mode);
 }
 
@@ -1008,7 +1010,7 @@ int Engine::run_graph(const std::vector<PieceRef>& refs, int policy, int mode, i
   if (it == graphs_.end()) {
     // Every buffer the captured launches use is created before capture (an
     // allocation during capture is not allowed).
-    if ((st = prefetch_tiles(refs, mode))) return st;
+    if ((st = prefetch_tiles(refs, policy, mode))) return st;
     const bool timing = timing_;
     timing_ = false;
     CKT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -1039,8 +1041,8 @@ int Engine::set_option(int opt, int64_t v) {
       steps_ = (int)v;
       return CKT_OK;
     case OPT_CNC_BLOCK:
-      if (v != 64 && v != 128 && v != 256 && v != 512)
-        return fail(CKT_ERR_INVALID_ARG, "block must be 64, 128, 256 or 512");
+      if (v != -1 && v != 64 && v != 128 && v != 256 && v != 512)
+        return fail(CKT_ERR_INVALID_ARG, "block must be 64, 128, 256, 512 or -1");
       cnc_block_ = (int)v;
       return CKT_OK;
     case OPT_PERSISTENT: persistent_ = v < 0 ? -1 : v ? 1 : 0; return CKT_OK;

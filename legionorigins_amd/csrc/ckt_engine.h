@@ -178,7 +178,12 @@ class Engine {
   int ensure_table();
   int tiles_for(int kind, const std::vector<PieceRef>& refs, int block, DevBuf* out);
   int upload_buf(uint64_t key, const void* data, size_t bytes, uint32_t n, DevBuf* out);
-  int prefetch_tiles(const std::vector<PieceRef>& refs, int mode);
+  int prefetch_tiles(const std::vector<PieceRef>& refs, int policy, int mode);
+  // calc_new_currents' workgroup size: OPT_CNC_BLOCK, or with -1 (default) 64
+  // for EXACT and 256 for the other policies.
+  int cnc_block_for(int policy) const {
+    return cnc_block_ > 0 ? cnc_block_ : (policy == CKT_POLICY_EXACT ? 64 : 256);
+  }
   // loop_once's launches; every tile list must already be cached.
   int launch_loop(const std::vector<PieceRef>& refs, int policy, int mode);
   void drop_caches();
@@ -216,7 +221,7 @@ class Engine {
   float* d_partial_ = nullptr;  // split LDS reduction's partial sums (grow-only)
   uint64_t partial_cap_ = 0;    // floats
   int steps_ = CKT_STEPS;
-  int cnc_block_ = 256;
+  int cnc_block_ = -1;       // -1: chosen per policy (cnc_block_for)
   int persistent_ = -1;      // -1: chosen per policy (launch_shape)
   int waves_per_simd_ = -1;  // -1: chosen per policy
   int dsc_atomic_impl_ = 0;
